@@ -1,0 +1,155 @@
+"""Benchmark of the per-cycle compute path on one MI355X.
+
+Workload (default): a Plummer sphere of N = 65,536 bodies (a = 1, M = 1, G = 1), FP32 pairwise
+force with Kahan tile sums, FP64 integration, eps = 0.01, dt = 1e-3; collisions off for the
+headline rate, then the same run with elastic collisions (radius 1e-4).  A "step" is one full
+cycle: force (+ detection), resolution, update.  Inputs are resident in HBM before timing.
+
+Prints one JSON line: G interactions/s (ordered pairs N(N-1) per step, the reference's
+nbody_computations counter), steps/s, the force kernel's FP32 roofline fraction measured with
+HIP events on the library's stream, the force error against an FP64 direct sum, and the
+reference's CPU algorithm (worker pool of threads over contiguous target slices) timed on a
+bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "G interactions/s + steps/s, plain vs surrogate at ≤1e-3 rel force err"
+PEAK_FP32 = 157.3e12  # MI355X FP32 vector peak (FLOP/s)
+FLOPS_PER_INTERACTION = 20
+
+WORKLOADS = {
+    "plummer65k": dict(n=65536, desc="Plummer sphere N=65536 (a=1, M=1, G=1), eps=0.01, dt=1e-3, "
+                                     "FP32 force + Kahan, FP64 update; collisions off, then elastic "
+                                     "radius 1e-4"),
+    "plummer262k": dict(n=262144, desc="Plummer sphere N=262144 (a=1, M=1, G=1), eps=0.01, "
+                                       "dt=1e-3, FP32 force + Kahan, FP64 update, collisions off"),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="plummer65k", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="use the default kernel shape")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def force_error(g, b, nsample=4096, seed=0):
+    """FP32-path force error against the device FP64 direct sum (which the tests pin to the CPU
+    oracle) on sampled targets: global ||da||/||a|| and per-body percentiles."""
+    idx = np.sort(np.random.default_rng(seed).choice(b.n, min(nsample, b.n), replace=False))
+    a32 = g.forces()[idx] / b.m[idx, None]
+    a64 = g.accel_fp64(idx)
+    d = np.linalg.norm(a32 - a64, axis=1)
+    per = d / np.linalg.norm(a64, axis=1)
+    return {"global": float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64)),
+            "p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99)),
+            "max": float(per.max()), "targets": int(idx.size),
+            "against": "FP64 direct sum on the device (bit-identical to the C oracle in tests)"}
+
+
+def timed_run(g, b, steps, warmup, dt=1e-3):
+    g.load(b)
+    if warmup:
+        g.step(dt, 1.0, nsteps=warmup)
+    g.set_timing(True)
+    t0 = time.perf_counter()
+    s = g.step(dt, 1.0, nsteps=steps)
+    t1 = time.perf_counter()
+    g.set_timing(False)
+    return s, t1 - t0
+
+
+def cpu_baseline(b, seconds):
+    """The reference's worker-pool CPU algorithm (oracle restatement) on a bounded sample: the
+    first T targets of the same body set against all N sources, one Compute phase."""
+    from oracle import binding as ob
+    threads = max(1, min(16, os.cpu_count() or 1))
+    st = ob.State(b.x, b.y, b.z, b.vx, b.vy, b.vz, b.m, b.r)
+    t = min(b.n, 256)
+    el, _, _ = ob.bench_workpool(st, 1.0, t, nsteps=1, nthreads=threads)
+    rate = t * (b.n - 1) / el
+    t = int(min(b.n, max(threads, rate * seconds / (b.n - 1))))
+    el, _, _ = ob.bench_workpool(st, 1.0, t, nsteps=1, nthreads=threads)
+    inter = t * (b.n - 1)
+    return {"value": inter / el / 1e9, "unit": "G interactions/s", "cores": threads,
+            "kind": "port",
+            "sample": "reference worker-pool algorithm (AoS bodies via pointers, force loop + "
+                      "collision-detection loop per target, %d threads over contiguous slices), "
+                      "%d targets x %d sources, one Compute phase, %.1f s" % (threads, t, b.n, el),
+            "seconds": el}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (per-GPU sharding is described in "
+              "DESIGN.md); got --gpus %d" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    from nbodygo_amd import NBodyGPU, gen
+
+    w = WORKLOADS[a.workload]
+    n = w["n"]
+    b = gen.plummer(n, seed=2024)
+    g = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=False)
+    g.load(b)
+    shape = g.config() if a.no_tune else g.autotune(reps=3)
+    s, el = timed_run(g, b, a.steps, a.warmup)
+    inter = s["interactions"]
+    ms_step = el / a.steps * 1e3
+    launch_ms = s["force_ms"] / max(s["force_launches"], 1)
+    per_launch = n * (n - 1)
+    achieved = FLOPS_PER_INTERACTION * per_launch / (launch_ms * 1e-3)
+    err = force_error(g, b)
+    out = {
+        "metric": METRIC,
+        "value": inter / el / 1e9,
+        "unit": "G interactions/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "steps_per_s": 1e3 / ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded Plummer sphere)",
+        "config": {"workload": w["desc"], "n": n, "eps": 0.01, "dt": 1e-3, "G": 1.0,
+                   "collisions": "off", "kernel_shape": shape},
+        "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "k_force_scalar/k_force_lds (FP32 pairwise force)",
+                     "flops_per_interaction": FLOPS_PER_INTERACTION,
+                     "interactions_per_launch": per_launch, "avg_launch_ms": launch_ms},
+        "force_error": err,
+    }
+    if a.workload == "plummer65k":
+        ge = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=True, collisions=True)
+        be = gen.plummer(n, seed=2024, radius=1e-4)
+        ge.load(be)
+        if not a.no_tune:
+            ge.autotune(reps=3)
+        se, ele = timed_run(ge, be, a.steps, a.warmup)
+        out["elastic"] = {"value": se["interactions"] / ele / 1e9, "unit": "G interactions/s",
+                          "ms_per_step": ele / a.steps * 1e3, "pairs_total": se["pairs_total"],
+                          "fixup_tiles": se["fixup_tiles"], "radius": 1e-4}
+        ge.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(b, a.cpu_seconds)
+    g.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

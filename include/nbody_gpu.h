@@ -1,0 +1,175 @@
+/*
+ * nbody_gpu.h -- C-ABI of the MI355X-native per-cycle compute path of aceeric/nbodygo.
+ *
+ * One call of nb_step() replaces, for every cycle it runs, the section of
+ * ComputationRunner.runOneComputation from BuildTree to the Update loop
+ * (cmd/runner/computation-runner.go:294-320):
+ *   - the worker-pool fan-out of Body.Compute        (workpool.go:82-115, body.go:148-187)
+ *     = O(N^2) gravity (calcForceFrom, body.go:214-225) + collision detection
+ *       (Collided, body.go:192-208),
+ *   - BodyCollection.ProcessMods                     (body_collection.go:212-233)
+ *     = elastic collision resolution (ResolveCollision body.go:248-264,
+ *       calcElasticCollision collisioncalc.go:42-186, doElastic collisioncalc.go:26-35),
+ *   - the Body.Update loop                           (body.go:114-139).
+ * The caller (the runner) keeps control polls, adds/deletes/mods, the result queue and
+ * BodyCollection.Cycle.
+ *
+ * Conventions: plain C types and pointers only; every function returns NB_OK (0) or a negative
+ * NB_E_* code, with a message from nb_last_error(); no exception crosses the boundary; the
+ * caller keeps ownership of every pointer it passes and nothing is retained after return (so a
+ * cgo caller can pass Go memory).  One context per caller thread; calls on one context are not
+ * re-entrant.  Device memory is owned by the context.
+ */
+#ifndef NBODY_GPU_H
+#define NBODY_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------------------- */
+#define NB_OK 0
+#define NB_E_INVALID (-1)       /* bad argument */
+#define NB_E_HIP (-2)           /* HIP runtime error (message has the HIP error string) */
+#define NB_E_NOMEM (-3)         /* device or host allocation failed */
+#define NB_E_PAIR_OVERFLOW (-4) /* more collision pairs than pair_capacity in some step; the
+                                   step completed, pairs beyond capacity were dropped (the
+                                   reference drops events beyond its 1000-slot channel,
+                                   body_collection.go:82-88) */
+#define NB_E_STATE (-5)         /* call out of order (e.g. nb_step before nb_load) */
+#define NB_E_UNSUPPORTED (-6)   /* option outside the ported path */
+
+/* ---- body flags and collision behaviour (Body fields, body.go:34-52; globals.go:11-16) ---- */
+#define NB_F_EXISTS 1u
+#define NB_F_FRAGMENTING 2u
+#define NB_F_SUN 4u
+#define NB_F_PINNED 8u
+
+#define NB_CB_NONE 0
+#define NB_CB_SUBSUME 1
+#define NB_CB_ELASTIC 2
+#define NB_CB_FRAGMENT 3
+
+/* ---- precision modes ---------------------------------------------------------------------- */
+#define NB_PREC_FP32 0 /* FP32 pairwise force with per-tile Kahan summation, FP64 exact
+                          overlap/collision tests, FP64 integration (production path) */
+#define NB_PREC_FP64 1 /* FP64 force in the reference's operation order (bit-identical to the
+                          reference's arithmetic for the force and update) */
+
+typedef struct nb_params {
+  double G;               /* gravitational constant; the reference hard-wires 6.673e-11
+                             (body.go:19) */
+  double eps;             /* Plummer softening length; 0 = the reference's unsoftened O(N^2)
+                             force (body.go:214-225) */
+  int32_t overlap_cutoff; /* 1 = pairs with d <= Ri+Rj contribute no force (body.go:219) */
+  int32_t collisions;     /* 1 = detect (body.go:171-186) and resolve elastic collisions */
+  int32_t precision;      /* NB_PREC_* */
+  int32_t device;         /* HIP device ordinal */
+  int64_t pair_capacity;  /* max collision pairs per step; 0 = default (max(4N, 65536)) */
+  int32_t ipl;            /* FP32 force kernel: target bodies per lane (1,2,4); 0 = tuned */
+  int32_t tile_j;         /* FP32 force kernel: j bodies per Kahan tile (multiple of 64);
+                             0 = tuned */
+  int32_t jsplit;         /* FP32 force kernel: source-range slices; 0 = tuned */
+  int32_t block;          /* FP32 force kernel: threads per workgroup (64..256); 0 = tuned */
+  int32_t variant;        /* FP32 force kernel: 0 = tuned, 1 = scalar-cache j stream,
+                             2 = LDS-staged j tiles */
+  int32_t reserved[7];
+} nb_params;
+
+typedef struct nb_stats {
+  int64_t steps;             /* cycles run by this call */
+  int64_t n;                 /* bodies in the context (existing or not) */
+  int64_t n_active;          /* existing, non-fragmenting bodies after the call */
+  double interactions;       /* ordered pairs evaluated by the force phase, summed over the
+                                call's steps (the reference's nbody_computations counter,
+                                body.go:166) */
+  int64_t pairs_total;       /* detected unordered collision pairs, summed over steps */
+  int64_t pairs_last;        /* detected unordered collision pairs in the last step */
+  int64_t resolved_total;    /* pairs for which calcElasticCollision reported a collision */
+  int64_t fragment_requests; /* shouldFragment() true (fragmentation is outside the path) */
+  int64_t removed_total;     /* bodies set !Exists by the NaN guard (body.go:134-137) */
+  int64_t fixup_tiles;       /* (target, tile) blocks sent to the FP64 exact pass */
+  int32_t pair_overflow;     /* 1 if some step exceeded pair_capacity */
+  int32_t pad0;
+  double wall_ms;            /* host wall time of the call */
+  double force_ms;           /* HIP-event time of the force kernels (timing enabled only) */
+  int64_t force_launches;    /* force kernel launches timed */
+} nb_stats;
+
+typedef struct nb_diag {
+  double kinetic;     /* sum 1/2 m v^2 over existing bodies */
+  double potential;   /* -G sum_{i<j} mi mj / sqrt(d^2 + eps^2) */
+  double momentum[3]; /* sum m v */
+  double mass;        /* sum m */
+  int64_t n_active;
+} nb_diag;
+
+typedef struct nb_ctx nb_ctx;
+
+/* Defaults: G = 6.673e-11, eps = 0, overlap_cutoff = 1, collisions = 1, FP32, device 0 --
+ * the reference's --no-barnes-hut semantics (cmd/server/main.go:221-222). */
+int nb_params_init(nb_params *p);
+
+/* Create a context on p->device.  On failure *out is NULL and nb_last_error(NULL) has the
+ * message. */
+int nb_create(const nb_params *p, nb_ctx **out);
+void nb_destroy(nb_ctx *ctx);
+
+/* Last error message of ctx (or of the last failed nb_create when ctx is NULL). */
+const char *nb_last_error(const nb_ctx *ctx);
+
+/* Replace the body array (BodyCollection.arr after Cycle, body_collection.go:253-296).
+ * x..r: FP64 per body.  beh: CollisionBehavior per body.  flags: NB_F_* per body.
+ * rest: per-body coefficient of restitution Body.r (NULL = 1.0, as NewBody sets, body.go:80).
+ * frag: per-body Body.FragFactor (NULL = 0).  Copies into device SoA; nothing retained. */
+int nb_load(nb_ctx *ctx, int64_t n, const double *x, const double *y, const double *z,
+            const double *vx, const double *vy, const double *vz, const double *m,
+            const double *r, const int32_t *beh, const uint8_t *flags, const double *rest,
+            const double *frag);
+
+/* Run nsteps cycles: force + detection, resolution in canonical pair order, FP64 update with
+ * time step dt (Runner.timeScaling) and restitution R (Runner.R).  Synchronous.  out may be
+ * NULL. */
+int nb_step(nb_ctx *ctx, double dt, double R, int32_t nsteps, nb_stats *out);
+
+/* Force phase only, on the current state (Body.fx/fy/fz after Compute): force components, not
+ * acceleration.  Also records the detected pairs for nb_get_pairs.  No resolution, no update. */
+int nb_compute_forces(nb_ctx *ctx, double *fx, double *fy, double *fz);
+
+/* FP64 accelerations sum_j G mj (xj - xi) / (d^2 + eps^2)^(3/2) for the nidx targets in idx,
+ * computed on the device in FP64 (with the context's overlap cutoff).  Used to state the FP32
+ * path's force error against FP64. */
+int nb_accel_fp64(nb_ctx *ctx, const int64_t *idx, int64_t nidx, double *ax, double *ay,
+                  double *az);
+
+/* Collision pairs detected by the last force phase, canonical order (i < j, ascending),
+ * as 2*n int32 body indices.  *n receives the count; at most cap pairs are written. */
+int nb_get_pairs(nb_ctx *ctx, int32_t *ij, int64_t cap, int64_t *n);
+
+/* Copy the FP64 state back (any pointer may be NULL to skip it). */
+int nb_store(nb_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+             uint8_t *flags, double *rest);
+
+/* FP32 positions for rendering (NewRenderable, renderable.go:22-40): xyz is 3*n floats;
+ * exists (n bytes, may be NULL) receives the Exists flag. */
+int nb_store_render(nb_ctx *ctx, float *xyz, uint8_t *exists);
+
+/* FP64 energy and momentum diagnostics (no reference counterpart; used for drift reports). */
+int nb_diagnostics(nb_ctx *ctx, nb_diag *out);
+
+/* Enable (1) or disable (0) HIP-event timing of the force kernels in nb_step. */
+int nb_set_timing(nb_ctx *ctx, int32_t enable);
+
+/* The FP32 force kernel shape in use: ipl, tile_j, jsplit, block, variant (5 ints). */
+int nb_get_config(nb_ctx *ctx, int32_t *cfg5);
+
+/* Measure the FP32 force kernel on the loaded state over a built-in candidate set of shapes
+ * (ipl x tile_j x jsplit x block x variant), keep the fastest for this context, and write it to
+ * cfg5 (may be NULL).  reps = timed launches per candidate. */
+int nb_autotune(nb_ctx *ctx, int32_t reps, int32_t *cfg5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

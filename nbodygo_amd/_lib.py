@@ -1,0 +1,139 @@
+"""ctypes binding of libnbgpu.so (include/nbody_gpu.h).
+
+The library is the only compute path: if it is missing, importing this module raises.  There is
+no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnbgpu.so")
+
+NB_OK = 0
+NB_E_INVALID = -1
+NB_E_HIP = -2
+NB_E_NOMEM = -3
+NB_E_PAIR_OVERFLOW = -4
+NB_E_STATE = -5
+NB_E_UNSUPPORTED = -6
+
+NB_F_EXISTS = 1
+NB_F_FRAGMENTING = 2
+NB_F_SUN = 4
+NB_F_PINNED = 8
+
+NB_CB_NONE = 0
+NB_CB_SUBSUME = 1
+NB_CB_ELASTIC = 2
+NB_CB_FRAGMENT = 3
+
+NB_PREC_FP32 = 0
+NB_PREC_FP64 = 1
+
+
+class NbParams(C.Structure):
+    _fields_ = [
+        ("G", C.c_double),
+        ("eps", C.c_double),
+        ("overlap_cutoff", C.c_int32),
+        ("collisions", C.c_int32),
+        ("precision", C.c_int32),
+        ("device", C.c_int32),
+        ("pair_capacity", C.c_int64),
+        ("ipl", C.c_int32),
+        ("tile_j", C.c_int32),
+        ("jsplit", C.c_int32),
+        ("block", C.c_int32),
+        ("variant", C.c_int32),
+        ("reserved", C.c_int32 * 7),
+    ]
+
+
+class NbStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("n", C.c_int64),
+        ("n_active", C.c_int64),
+        ("interactions", C.c_double),
+        ("pairs_total", C.c_int64),
+        ("pairs_last", C.c_int64),
+        ("resolved_total", C.c_int64),
+        ("fragment_requests", C.c_int64),
+        ("removed_total", C.c_int64),
+        ("fixup_tiles", C.c_int64),
+        ("pair_overflow", C.c_int32),
+        ("pad0", C.c_int32),
+        ("wall_ms", C.c_double),
+        ("force_ms", C.c_double),
+        ("force_launches", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad0"}
+
+
+class NbDiag(C.Structure):
+    _fields_ = [
+        ("kinetic", C.c_double),
+        ("potential", C.c_double),
+        ("momentum", C.c_double * 3),
+        ("mass", C.c_double),
+        ("n_active", C.c_int64),
+    ]
+
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u8p = C.POINTER(C.c_uint8)
+_ctx = C.c_void_p
+
+# (name, restype, argtypes): every symbol declared in include/nbody_gpu.h
+SIGNATURES = [
+    ("nb_params_init", C.c_int, [C.POINTER(NbParams)]),
+    ("nb_create", C.c_int, [C.POINTER(NbParams), C.POINTER(_ctx)]),
+    ("nb_destroy", None, [_ctx]),
+    ("nb_last_error", C.c_char_p, [_ctx]),
+    ("nb_load", C.c_int,
+     [_ctx, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _u8p, _dp, _dp]),
+    ("nb_step", C.c_int, [_ctx, C.c_double, C.c_double, C.c_int32, C.POINTER(NbStats)]),
+    ("nb_compute_forces", C.c_int, [_ctx, _dp, _dp, _dp]),
+    ("nb_accel_fp64", C.c_int, [_ctx, _i64p, C.c_int64, _dp, _dp, _dp]),
+    ("nb_get_pairs", C.c_int, [_ctx, _i32p, C.c_int64, _i64p]),
+    ("nb_store", C.c_int, [_ctx, _dp, _dp, _dp, _dp, _dp, _dp, _u8p, _dp]),
+    ("nb_store_render", C.c_int, [_ctx, _fp, _u8p]),
+    ("nb_diagnostics", C.c_int, [_ctx, C.POINTER(NbDiag)]),
+    ("nb_set_timing", C.c_int, [_ctx, C.c_int32]),
+    ("nb_get_config", C.c_int, [_ctx, _i32p]),
+    ("nb_autotune", C.c_int, [_ctx, C.c_int32, _i32p]),
+]
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError(
+            "libnbgpu.so not built at %s: run __graft_entry__.build() (there is no CPU fallback)"
+            % path)
+    lib = C.CDLL(path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = load_library()
+
+
+class NBodyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("nb error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc, ctx=None):
+    if rc != NB_OK:
+        msg = lib.nb_last_error(ctx)
+        raise NBodyError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,606 @@
+// nb_cycle.hip -- the rest of one compute cycle on gfx950: FP64 reference-order force, pair sort,
+// elastic collision resolution, FP64 update, source-stream maintenance, diagnostics.
+//
+// Compiled with -ffp-contract=off: every FP64 expression below is evaluated as written, one
+// correctly rounded IEEE operation at a time (gfx950's FP64 divide and square root are correctly
+// rounded), which is what makes the FP64 path bit-identical to the reference's arithmetic.
+#include <cfloat>
+#include <cmath>
+
+#include "nb_internal.hpp"
+
+namespace nb {
+
+typedef const __attribute__((address_space(4))) double *cdp;
+typedef const __attribute__((address_space(4))) int32_t *cip;
+
+__device__ __forceinline__ bool active_f(uint8_t f) {
+  return (f & NB_F_EXISTS) && !(f & NB_F_FRAGMENTING);
+}
+__device__ __forceinline__ bool collide_b(int32_t b) {
+  return b == NB_CB_ELASTIC || b == NB_CB_FRAGMENT;
+}
+
+__device__ static void emit(uint64_t *pairs, DevCounters *ctr, int64_t cap, int i, int j) {
+  uint32_t q = atomicAdd(&ctr->pair_count, 1u);
+  if ((int64_t)q < cap)
+    pairs[q] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+  else
+    atomicOr(&ctr->pair_overflow, 1u);
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 force in the reference's operation order, one thread per target, sources in index order.
+//   eps == 0 && cutoff: calcForceFrom (body.go:214-225)
+//       dist = sqrt(dx*dx+dy*dy+dz*dz); if dist > Ri+Rj: force = G*mi*mj/(dist*dist);
+//       fx += force*dx/dist
+//   otherwise: directForce (octree.go:314-325) with optional gate on the unsoftened distance
+//       distSq = d2 + eps*eps; dist = sqrt(distSq); force = G*mi*mj/distSq; fx += force*dx/dist
+// Pairs are emitted from the lower index (Collided, body.go:192-208).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_force_f64(Force64Args a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.npad) return;
+  double fx = 0, fy = 0, fz = 0;
+  const bool ai = i < a.n && active_f(a.flags[i]);
+  if (__builtin_amdgcn_ballot_w64(ai) != 0) {
+    const double xi = ai ? a.x[i] : 0, yi = ai ? a.y[i] : 0, zi = ai ? a.z[i] : 0;
+    const double mi = ai ? a.m[i] : 0, ri = ai ? a.r[i] : 0;
+    const bool ci = ai && collide_b(a.beh[i]);
+    const double G = a.G, eps2 = a.eps * a.eps;
+    const bool ref = a.eps == 0.0 && a.cutoff;
+    cdp X = (cdp)a.x, Y = (cdp)a.y, Z = (cdp)a.z, M = (cdp)a.m, Rr = (cdp)a.r;
+    cip B = (cip)a.beh;
+    for (int j = 0; j < a.n; j++) {
+      if (!active_f(a.flags[j])) continue;
+      const double dx = X[j] - xi;
+      const double dy = Y[j] - yi;
+      const double dz = Z[j] - zi;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double rs = ri + Rr[j];
+      if (j != i && ai) {
+        bool overlap;
+        if (ref) {
+          const double dist = __builtin_sqrt(d2);
+          overlap = !(dist > rs);
+          if (!overlap) {
+            const double force = G * mi * M[j] / (dist * dist);
+            fx += force * dx / dist;
+            fy += force * dy / dist;
+            fz += force * dz / dist;
+          }
+        } else {
+          overlap = !(__builtin_sqrt(d2) > rs);
+          if (!(a.cutoff && overlap)) {
+            const double distSq = d2 + eps2;
+            const double dist = __builtin_sqrt(distSq);
+            const double force = G * mi * M[j] / distSq;
+            fx += force * dx / dist;
+            fy += force * dy / dist;
+            fz += force * dz / dist;
+          }
+        }
+        if (a.collisions && overlap && i < j && ci && collide_b(B[j]))
+          emit(a.pairs, a.ctr, a.pair_cap, i, j);
+      }
+    }
+  }
+  a.acc[i] = fx;
+  a.acc[a.npad + i] = fy;
+  a.acc[2 * a.npad + i] = fz;
+}
+
+hipError_t launch_force_f64(const Force64Args &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_force_f64, dim3((unsigned)(a.npad / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// FP64 accelerations for a sample of targets (error reference for the FP32 path).
+__global__ __launch_bounds__(256) void k_accel_f64(Accel64Args a) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nidx) return;
+  const int64_t i = a.idx[k];
+  double ax = 0, ay = 0, az = 0;
+  if (i >= 0 && i < a.n && active_f(a.flags[i])) {
+    const double xi = a.x[i], yi = a.y[i], zi = a.z[i], ri = a.r[i];
+    const double eps2 = a.eps * a.eps;
+    for (int j = 0; j < a.n; j++) {
+      if (j == i || !active_f(a.flags[j])) continue;
+      const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (a.cutoff && !(__builtin_sqrt(d2) > ri + a.r[j])) continue;
+      const double ds = d2 + eps2;
+      const double w = a.G * a.m[j] / (ds * __builtin_sqrt(ds));
+      ax += w * dx;
+      ay += w * dy;
+      az += w * dz;
+    }
+  }
+  a.out[k] = ax;
+  a.out[a.nidx + k] = ay;
+  a.out[2 * a.nidx + k] = az;
+}
+
+hipError_t launch_accel_f64(const Accel64Args &a, hipStream_t st) {
+  if (a.nidx <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_accel_f64, dim3((unsigned)((a.nidx + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pair sort: keys (i << 32 | j) ascending = canonical order.  One workgroup; bitonic network in
+// LDS up to kSortLdsKeys keys, in global memory (padded to a power of two) beyond.
+// ---------------------------------------------------------------------------------------------
+// keys has room for the next power of two above cap.
+__global__ __launch_bounds__(1024) void k_sort_pairs(uint64_t *keys, int64_t cap,
+                                                     const DevCounters *ctr) {
+  __shared__ uint64_t sh[kSortLdsKeys];
+  const int64_t cnt = min((int64_t)ctr->pair_count, cap);
+  if (cnt <= 1) return;
+  int64_t P = 1;
+  while (P < cnt) P <<= 1;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (P <= kSortLdsKeys) {
+    for (int64_t q = tid; q < P; q += nt) sh[q] = q < cnt ? keys[q] : ~0ull;
+    __syncthreads();
+    for (int64_t k = 2; k <= P; k <<= 1) {
+      for (int64_t j = k >> 1; j > 0; j >>= 1) {
+        for (int64_t q = tid; q < P; q += nt) {
+          const int64_t o = q ^ j;
+          if (o > q) {
+            const uint64_t u = sh[q], v = sh[o];
+            const bool up = (q & k) == 0;
+            if ((u > v) == up) {
+              sh[q] = v;
+              sh[o] = u;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int64_t q = tid; q < cnt; q += nt) keys[q] = sh[q];
+  } else {
+    for (int64_t q = cnt + tid; q < P; q += nt) keys[q] = ~0ull;
+    __threadfence_block();
+    __syncthreads();
+    for (int64_t k = 2; k <= P; k <<= 1) {
+      for (int64_t j = k >> 1; j > 0; j >>= 1) {
+        for (int64_t q = tid; q < P; q += nt) {
+          const int64_t o = q ^ j;
+          if (o > q) {
+            const uint64_t u = keys[q], v = keys[o];
+            const bool up = (q & k) == 0;
+            if ((u > v) == up) {
+              keys[q] = v;
+              keys[o] = u;
+            }
+          }
+        }
+        __threadfence_block();
+        __syncthreads();
+      }
+    }
+  }
+}
+
+hipError_t launch_sort_pairs(uint64_t *keys, int64_t cap, const DevCounters *ctr,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_sort_pairs, dim3(1), dim3(1024), 0, st, keys, cap, ctr);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Elastic collision resolution (ProcessMods, body_collection.go:212-233), one thread walking the
+// sorted pair list: each unordered pair is resolved once, as lo.ResolveCollision(hi) when lo is
+// Elastic, else hi.ResolveCollision(lo) (only an Elastic first body resolves, body.go:252-253).
+// ---------------------------------------------------------------------------------------------
+struct Elastic {
+  double v1[3], v2[3], cm[3];
+};
+
+// calcElasticCollision, cmd/body/collisioncalc.go:42-186 (coll3d).
+__device__ static bool elastic(const ResolveArgs &a, int b, int o, Elastic &res) {
+  double r12, m21, d, v, theta2, phi2, st, ct, sp, cp, vx1r, vy1r, vz1r, fvz1r, thetav, phiv, dr,
+      alpha, beta, sbeta, cbeta, aa, dvz2, vx2r, vy2r, vz2r, x21, y21, z21, vx21, vy21, vz21,
+      vx_cm, vy_cm, vz_cm;
+  const double m1 = a.m[b], m2 = a.m[o];
+  const double r1 = a.r[b], r2 = a.r[o];
+  const double x1 = a.x[b], y1 = a.y[b], z1 = a.z[b];
+  double x2 = a.x[o], y2 = a.y[o], z2 = a.z[o];
+  double vx1 = a.vx[b], vy1 = a.vy[b], vz1 = a.vz[b];
+  const double vx2 = a.vx[o], vy2 = a.vy[o], vz2 = a.vz[o];
+
+  r12 = r1 + r2;
+  m21 = m2 / m1;
+  x21 = x2 - x1;
+  y21 = y2 - y1;
+  z21 = z2 - z1;
+  vx21 = vx2 - vx1;
+  vy21 = vy2 - vy1;
+  vz21 = vz2 - vz1;
+
+  vx_cm = (m1 * vx1 + m2 * vx2) / (m1 + m2);
+  vy_cm = (m1 * vy1 + m2 * vy2) / (m1 + m2);
+  vz_cm = (m1 * vz1 + m2 * vz2) / (m1 + m2);
+
+  d = sqrt(x21 * x21 + y21 * y21 + z21 * z21);
+  v = sqrt(vx21 * vx21 + vy21 * vy21 + vz21 * vz21);
+  if (v == 0) return false;
+
+  x2 = x21;
+  y2 = y21;
+  z2 = z21;
+  vx1 = -vx21;
+  vy1 = -vy21;
+  vz1 = -vz21;
+
+  theta2 = acos(z2 / d);
+  if (x2 == 0 && y2 == 0)
+    phi2 = 0;
+  else
+    phi2 = atan2(y2, x2);
+  st = sin(theta2);
+  ct = cos(theta2);
+  sp = sin(phi2);
+  cp = cos(phi2);
+
+  vx1r = ct * cp * vx1 + ct * sp * vy1 - st * vz1;
+  vy1r = cp * vy1 - sp * vx1;
+  vz1r = st * cp * vx1 + st * sp * vy1 + ct * vz1;
+  fvz1r = vz1r / v;
+  if (fvz1r > 1)
+    fvz1r = 1;
+  else if (fvz1r < -1)
+    fvz1r = -1;
+  thetav = acos(fvz1r);
+  if (vx1r == 0 && vy1r == 0)
+    phiv = 0;
+  else
+    phiv = atan2(vy1r, vx1r);
+
+  dr = d * sin(thetav) / r12;
+  if (thetav > M_PI / 2 || fabs(dr) > 1) return false;
+
+  alpha = asin(-dr);
+  beta = phiv;
+  sbeta = sin(beta);
+  cbeta = cos(beta);
+
+  aa = tan(thetav + alpha);
+  dvz2 = 2 * (vz1r + aa * (cbeta * vx1r + sbeta * vy1r)) / ((1 + aa * aa) * (1 + m21));
+
+  vz2r = dvz2;
+  vx2r = aa * cbeta * dvz2;
+  vy2r = aa * sbeta * dvz2;
+  vz1r = vz1r - m21 * vz2r;
+  vx1r = vx1r - m21 * vx2r;
+  vy1r = vy1r - m21 * vy2r;
+
+  res.v1[0] = ct * cp * vx1r - sp * vy1r + st * cp * vz1r + vx2;
+  res.v1[1] = ct * sp * vx1r + cp * vy1r + st * sp * vz1r + vy2;
+  res.v1[2] = ct * vz1r - st * vx1r + vz2;
+  res.v2[0] = ct * cp * vx2r - sp * vy2r + st * cp * vz2r + vx2;
+  res.v2[1] = ct * sp * vx2r + cp * vy2r + st * sp * vz2r + vy2;
+  res.v2[2] = ct * vz2r - st * vx2r + vz2;
+  res.cm[0] = vx_cm;
+  res.cm[1] = vy_cm;
+  res.cm[2] = vz_cm;
+  return true;
+}
+
+// shouldFragment (fragcalc.go:24-49).  Fragmentation itself is outside the ported path: a true
+// result is counted and the velocities are left unchanged.
+__device__ static bool should_fragment(const ResolveArgs &a, int b, int o, const Elastic &e) {
+  if (!(a.beh[b] == NB_CB_FRAGMENT || a.beh[o] == NB_CB_FRAGMENT)) return false;
+  const double br = a.rest[b];
+  const double vThis = a.vx[b] + a.vy[b] + a.vz[b];
+  const double dvThis = fabs(a.vx[b] - ((e.v1[0] - e.cm[0]) * br + e.cm[0])) +
+                        fabs(a.vy[b] - ((e.v1[1] - e.cm[1]) * br + e.cm[1])) +
+                        fabs(a.vz[b] - ((e.v1[2] - e.cm[2]) * br + e.cm[2]));
+  const double thisFactor = dvThis / fabs(vThis);
+  const double vOther = a.vx[o] + a.vy[o] + a.vz[o];
+  const double dvOther = fabs(a.vx[o] - ((e.v2[0] - e.cm[0]) * br + e.cm[0])) +
+                         fabs(a.vy[o] - ((e.v2[1] - e.cm[1]) * br + e.cm[1])) +
+                         fabs(a.vz[o] - ((e.v2[2] - e.cm[2]) * br + e.cm[2]));
+  const double otherFactor = dvOther / fabs(vOther);
+  const double fb = a.frag[b], fo = a.frag[o];
+  return (a.beh[b] == NB_CB_FRAGMENT && thisFactor > fb) ||
+         (a.beh[o] == NB_CB_FRAGMENT && otherFactor > fo);
+}
+
+__global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
+  if (threadIdx.x != 0) return;
+  const uint32_t found = a.ctr->pair_count;
+  const int64_t cnt = min((int64_t)found, a.pair_cap);
+  uint64_t resolved = 0, fragreq = 0;
+  for (int64_t k = 0; k < cnt; k++) {
+    const uint64_t key = a.pairs[k];
+    const int lo = (int)(key >> 32), hi = (int)(key & 0xffffffffu);
+    int b, o;
+    if (a.beh[lo] == NB_CB_ELASTIC && collide_b(a.beh[hi])) {
+      b = lo;
+      o = hi;
+    } else if (a.beh[hi] == NB_CB_ELASTIC && collide_b(a.beh[lo])) {
+      b = hi;
+      o = lo;
+    } else {
+      continue;
+    }
+    if (!(a.flags[b] & NB_F_EXISTS) || !(a.flags[o] & NB_F_EXISTS)) continue;
+    Elastic e;
+    if (!elastic(a, b, o, e)) continue;
+    resolved++;
+    if (should_fragment(a, b, o, e)) {
+      fragreq++;
+      continue;
+    }
+    // doElastic (collisioncalc.go:26-35): restitution of the first body.
+    const double rr = a.rest[b];
+    a.vx[b] = (e.v1[0] - e.cm[0]) * rr + e.cm[0];
+    a.vy[b] = (e.v1[1] - e.cm[1]) * rr + e.cm[1];
+    a.vz[b] = (e.v1[2] - e.cm[2]) * rr + e.cm[2];
+    a.vx[o] = (e.v2[0] - e.cm[0]) * rr + e.cm[0];
+    a.vy[o] = (e.v2[1] - e.cm[1]) * rr + e.cm[1];
+    a.vz[o] = (e.v2[2] - e.cm[2]) * rr + e.cm[2];
+    a.collided[b] = 1;
+    a.collided[o] = 1;
+  }
+  a.ctr->pairs_total += found;
+  a.ctr->pairs_last = found;
+  a.ctr->resolved_total += resolved;
+  a.ctr->fragment_requests += fragreq;
+}
+
+hipError_t launch_resolve(const ResolveArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_resolve, dim3(1), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Update: Body.Update (body.go:114-139) + rewrite of the FP32 source stream.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float4 stream_entry(bool act, double x, double y, double z, double m) {
+  return act ? make_float4((float)x, (float)y, (float)z, (float)m)
+             : make_float4(kSentinel, kSentinel, kSentinel, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_update(UpdateArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.npad) return;
+  if (i >= a.n) {
+    a.js[i] = stream_entry(false, 0, 0, 0, 0);
+    return;
+  }
+  uint8_t f = a.flags[i];
+  if (!(f & NB_F_EXISTS)) {
+    a.js[i] = stream_entry(false, 0, 0, 0, 0);
+    return;
+  }
+  const double dt = a.dt;
+  double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+  if (!a.collided[i]) {
+    const double mi = a.m[i];
+    double fx = 0, fy = 0, fz = 0;
+    for (int s = 0; s < a.split; s++) {
+      const double *o = a.acc + (size_t)s * 3 * a.npad;
+      fx += o[i];
+      fy += o[a.npad + i];
+      fz += o[2 * a.npad + i];
+    }
+    if (a.force_mode == 1) {
+      // FP64 path: acc holds the force; Vx += timeScaling * fx / Mass (body.go:120-122).
+      vx += dt * fx / mi;
+      vy += dt * fy / mi;
+      vz += dt * fz / mi;
+    } else {
+      // FP32 path: acc holds sum mj dx / r^3; the kick is dt * G * a.  A massless body gets the
+      // reference's 0/0 (its force G*0*mj/d^2 is zero), which the NaN guard below removes.
+      if (mi == 0.0) {
+        vx = vy = vz = __builtin_nan("");
+      } else {
+        vx += dt * (a.G * fx);
+        vy += dt * (a.G * fy);
+        vz += dt * (a.G * fz);
+      }
+    }
+    a.vx[i] = vx;
+    a.vy[i] = vy;
+    a.vz[i] = vz;
+  }
+  const double x = a.x[i] + dt * vx;
+  const double y = a.y[i] + dt * vy;
+  const double z = a.z[i] + dt * vz;
+  a.x[i] = x;
+  a.y[i] = y;
+  a.z[i] = z;
+  a.collided[i] = 0;
+  a.rest[i] = a.R;
+  if (isnan(x) || isnan(y) || isnan(z)) {
+    f &= (uint8_t)~NB_F_EXISTS;
+    a.flags[i] = f;
+    atomicAdd((unsigned long long *)&a.ctr->removed_total, 1ull);
+  }
+  a.js[i] = stream_entry(active_f(f), x, y, z, a.m[i]);
+}
+
+hipError_t launch_update(const UpdateArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_update, dim3((unsigned)(a.npad / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+__global__ void k_write_stream(const double *x, const double *y, const double *z,
+                               const double *m, const uint8_t *flags, float4 *js, int n,
+                               int npad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  js[i] = i < n ? stream_entry(active_f(flags[i]), x[i], y[i], z[i], m[i])
+                : stream_entry(false, 0, 0, 0, 0);
+}
+
+hipError_t launch_write_stream(const double *x, const double *y, const double *z,
+                               const double *m, const uint8_t *flags, float4 *js, int n,
+                               int npad, hipStream_t st) {
+  hipLaunchKernelGGL(k_write_stream, dim3((unsigned)(npad / 256)), dim3(256), 0, st, x, y, z, m,
+                     flags, js, n, npad);
+  return hipGetLastError();
+}
+
+// Per-tile max radius over active bodies, rounded up to the next float.
+__global__ __launch_bounds__(256) void k_rmax_tile(const double *r, const uint8_t *flags, int n,
+                                                   int tile_j, float *out) {
+  __shared__ double sh[256];
+  const int t = blockIdx.x;
+  double mx = 0;
+  for (int q = threadIdx.x; q < tile_j; q += blockDim.x) {
+    const int j = t * tile_j + q;
+    if (j < n && active_f(flags[j])) mx = fmax(mx, r[j]);
+  }
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float f = (float)sh[0];
+    if ((double)f < sh[0]) f = __uint_as_float(__float_as_uint(f) + 1u);
+    out[t] = f;
+  }
+}
+
+hipError_t launch_rmax_tile(const double *r, const uint8_t *flags, int n, int tile_j, int tiles,
+                            float *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rmax_tile, dim3((unsigned)tiles), dim3(256), 0, st, r, flags, n, tile_j,
+                     out);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_counters(DevCounters *c, int full) {
+  if (threadIdx.x != 0) return;
+  if (full) {
+    DevCounters z{};
+    *c = z;
+  } else {
+    c->pair_count = 0;
+  }
+}
+
+hipError_t launch_zero_counters(DevCounters *ctr, int full, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, ctr, full);
+  return hipGetLastError();
+}
+
+// Force per body (Body.fx/fy/fz) from the per-slice accumulators.
+__global__ void k_combine_force(const double *acc, int split, int npad, int n, const double *m,
+                                double G, int force_mode, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[3] = {0, 0, 0};
+  for (int s = 0; s < split; s++)
+    for (int c = 0; c < 3; c++) f[c] += acc[((size_t)s * 3 + c) * npad + i];
+  for (int c = 0; c < 3; c++)
+    out[(size_t)c * n + i] = force_mode == 1 ? f[c] : m[i] * (G * f[c]);
+}
+
+hipError_t launch_combine_force(const double *acc, int split, int npad, int n, const double *m,
+                                double G, int force_mode, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_combine_force, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, acc,
+                     split, npad, n, m, G, force_mode, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Diagnostics: per-body FP64 terms then a fixed-order tree reduction (deterministic).
+// scratch layout: [0, 8*n) per-body terms (ke, pe, px, py, pz, m, active, 0), then the result.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag_terms(const double *x, const double *y,
+                                                    const double *z, const double *vx,
+                                                    const double *vy, const double *vz,
+                                                    const double *m, const uint8_t *flags, int n,
+                                                    double eps, double *t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ke = 0, pe = 0, px = 0, py = 0, pz = 0, mm = 0, act = 0;
+  if (flags[i] & NB_F_EXISTS) {
+    const double mi = m[i];
+    ke = 0.5 * mi * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    px = mi * vx[i];
+    py = mi * vy[i];
+    pz = mi * vz[i];
+    mm = mi;
+    act = 1;
+    const double eps2 = eps * eps;
+    for (int j = 0; j < n; j++) {
+      if (j == i || !(flags[j] & NB_F_EXISTS)) continue;
+      const double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+      const double d2 = dx * dx + dy * dy + dz * dz + eps2;
+      if (d2 > 0) pe -= mi * m[j] / __builtin_sqrt(d2);
+    }
+    pe *= 0.5;
+  }
+  double *o = t + (size_t)i * 8;
+  o[0] = ke;
+  o[1] = pe;
+  o[2] = px;
+  o[3] = py;
+  o[4] = pz;
+  o[5] = mm;
+  o[6] = act;
+  o[7] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_diag_reduce(const double *t, int n, double G,
+                                                     double *out) {
+  __shared__ double sh[256][8];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // Fixed assignment of bodies to threads and a fixed tree: the result does not depend on
+  // scheduling.
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    for (int c = 0; c < 8; c++) acc[c] += t[(size_t)i * 8 + c];
+  for (int c = 0; c < 8; c++) sh[threadIdx.x][c] = acc[c];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 8; c++) sh[threadIdx.x][c] += sh[threadIdx.x + w][c];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < 8; c++) out[c] = sh[0][c];
+    out[1] = sh[0][1] * G;
+  }
+}
+
+hipError_t launch_diag(const double *x, const double *y, const double *z, const double *vx,
+                       const double *vy, const double *vz, const double *m, const uint8_t *flags,
+                       int n, double G, double eps, double *scratch, int64_t scratch_len,
+                       hipStream_t st) {
+  if (scratch_len < (int64_t)n * 8 + 8) return hipErrorInvalidValue;
+  if (n > 0)
+    hipLaunchKernelGGL(k_diag_terms, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, y,
+                       z, vx, vy, vz, m, flags, n, eps, scratch);
+  hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(256), 0, st, scratch, n, G,
+                     scratch + (size_t)n * 8);
+  return hipGetLastError();
+}
+
+__global__ void k_render(const double *x, const double *y, const double *z, const uint8_t *flags,
+                         int n, float *xyz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool e = flags[i] & NB_F_EXISTS;
+  // NewRenderable (renderable.go:22-40): float32 positions; zero for a non-existent body.
+  xyz[3 * (size_t)i] = e ? (float)x[i] : 0.f;
+  xyz[3 * (size_t)i + 1] = e ? (float)y[i] : 0.f;
+  xyz[3 * (size_t)i + 2] = e ? (float)z[i] : 0.f;
+}
+
+hipError_t launch_render(const double *x, const double *y, const double *z, const uint8_t *flags,
+                         int n, float *xyz, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_render, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, y, z,
+                     flags, n, xyz);
+  return hipGetLastError();
+}
+
+}  // namespace nb

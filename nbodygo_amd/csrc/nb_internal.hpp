@@ -1,0 +1,202 @@
+// nb_internal.hpp -- device data layout, context and launch helpers shared by the HIP sources of
+// libnbgpu.so (the MI355X per-cycle compute path of aceeric/nbodygo).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nbody_gpu.h"
+
+namespace nb {
+
+// ---- fixed shapes ---------------------------------------------------------------------------
+constexpr int kPad = 1024;           // body arrays are padded to a multiple of this
+constexpr int kFlagSlots = 4;        // flagged tiles remembered per (slice, target)
+constexpr int kMaxSplit = 64;        // max source-range slices
+constexpr int kSortLdsKeys = 8192;   // pair keys sorted in LDS by one workgroup
+constexpr float kSentinel = 1.0e18f; // position of padding / inactive sources in the FP32 stream
+
+// Device counters, one struct per context (zeroed by a fill kernel, never by hipMemset, so the
+// step sequence stays capturable).
+struct DevCounters {
+  uint32_t pair_count;       // pairs appended in the current force phase
+  uint32_t fixup_count;      // unused slot kept for alignment
+  uint64_t pairs_total;
+  uint64_t pairs_last;
+  uint64_t resolved_total;
+  uint64_t fragment_requests;
+  uint64_t removed_total;
+  uint64_t fixup_tiles;
+  uint32_t pair_overflow;
+  uint32_t pad;
+};
+
+// FP32 force kernel launch shape.
+struct ForceShape {
+  int ipl = 2;      // target bodies per lane
+  int tile_j = 256; // sources per Kahan tile
+  int jsplit = 1;   // source-range slices (grid.y)
+  int block = 256;  // threads per workgroup
+  int variant = 1;  // 1 = scalar-cache j stream, 2 = LDS-staged tiles
+};
+
+struct Ctx {
+  nb_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int64_t n = 0;     // bodies
+  int64_t npad = 0;  // padded count (multiple of kPad)
+  int64_t cap = 0;   // allocated capacity (bodies)
+  int64_t n_active = 0;
+  bool loaded = false;
+
+  // FP64 state (SoA)
+  double *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  double *m = nullptr, *r = nullptr, *rest = nullptr, *frag = nullptr;
+  int32_t *beh = nullptr;
+  uint8_t *flags = nullptr, *collided = nullptr;
+
+  // FP32 source stream (x, y, z, m) and per-tile max radius
+  float4 *js = nullptr;
+  float *rmax_tile = nullptr;
+  int rmax_tile_j = 0;  // tile size rmax_tile was built for
+
+  // per-slice accelerations (FP32 path) or force (FP64 path): [split][3][npad] doubles
+  double *acc = nullptr;
+  int acc_split = 0;
+  // flagged tiles: counts [split][npad], lists [split][npad][kFlagSlots]
+  int32_t *fcnt = nullptr, *flist = nullptr;
+
+  // collision pairs as 64-bit keys (i << 32 | j)
+  uint64_t *pairs = nullptr;
+  int64_t pair_cap = 0;
+  int64_t pair_cap_pow2 = 0;
+
+  DevCounters *ctr = nullptr;
+  double *diag = nullptr;  // scratch for reductions
+  int64_t diag_len = 0;
+
+  ForceShape shape;
+  bool shape_tuned = false;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+};
+
+bool hip_ok(Ctx *c, hipError_t e, const char *what);
+#define NB_HIP(c, call)                                  \
+  do {                                                   \
+    if (!::nb::hip_ok((c), (call), #call)) return NB_E_HIP; \
+  } while (0)
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// ---- kernels launched from nb_api.cpp (defined in the .hip files) ----------------------------
+struct ForceArgs {
+  const float4 *js;
+  const float *rmax_tile;
+  const double *r;
+  double *acc;
+  int32_t *fcnt;
+  int32_t *flist;
+  int npad;
+  int n;
+  int tiles;            // npad / tile_j
+  int tiles_per_slice;
+  int tile_j;
+  float eps2;
+  int detect;           // track per-tile minimum distance and flag tiles
+  int drop_flagged;     // flagged tiles' FP32 partials are discarded (recomputed in FP64)
+};
+hipError_t launch_force_f32(const ForceShape &s, const ForceArgs &a, hipStream_t st);
+
+struct FixupArgs {
+  const double *x, *y, *z, *m, *r;
+  const int32_t *beh;
+  const uint8_t *flags;
+  double *acc;
+  const int32_t *fcnt;
+  const int32_t *flist;
+  uint64_t *pairs;
+  DevCounters *ctr;
+  int64_t pair_cap;
+  int npad, n, tiles, tiles_per_slice, tile_j, split;
+  double eps2;
+  int recompute;  // add FP64 contributions of flagged tiles (the fast path dropped them)
+  int cutoff;     // overlap gate
+  int collisions; // emit pairs
+};
+hipError_t launch_fixup(const FixupArgs &a, hipStream_t st);
+
+struct Force64Args {
+  const double *x, *y, *z, *m, *r;
+  const int32_t *beh;
+  const uint8_t *flags;
+  double *acc;  // force [3][npad]
+  uint64_t *pairs;
+  DevCounters *ctr;
+  int64_t pair_cap;
+  int n, npad;
+  double G, eps;
+  int cutoff, collisions;
+};
+hipError_t launch_force_f64(const Force64Args &a, hipStream_t st);
+
+struct Accel64Args {
+  const double *x, *y, *z, *m, *r;
+  const uint8_t *flags;
+  const int64_t *idx;
+  int64_t nidx;
+  double *out;  // [3][nidx]
+  int n;
+  double G, eps;
+  int cutoff;
+};
+hipError_t launch_accel_f64(const Accel64Args &a, hipStream_t st);
+
+hipError_t launch_sort_pairs(uint64_t *keys, int64_t cap, const DevCounters *ctr,
+                             hipStream_t st);
+
+struct ResolveArgs {
+  double *x, *y, *z, *vx, *vy, *vz, *m, *r, *rest, *frag;
+  const int32_t *beh;
+  uint8_t *flags, *collided;
+  const uint64_t *pairs;
+  DevCounters *ctr;
+  int64_t pair_cap;
+};
+hipError_t launch_resolve(const ResolveArgs &a, hipStream_t st);
+
+struct UpdateArgs {
+  double *x, *y, *z, *vx, *vy, *vz;
+  const double *m;
+  double *rest;
+  uint8_t *flags, *collided;
+  const double *acc;
+  int split, npad, n;
+  double G, dt, R;
+  int force_mode;  // 0: acc holds accelerations / G (FP32 path); 1: acc holds force (FP64 path)
+  float4 *js;
+  DevCounters *ctr;
+};
+hipError_t launch_update(const UpdateArgs &a, hipStream_t st);
+
+hipError_t launch_write_stream(const double *x, const double *y, const double *z,
+                               const double *m, const uint8_t *flags, float4 *js, int n,
+                               int npad, hipStream_t st);
+hipError_t launch_rmax_tile(const double *r, const uint8_t *flags, int n, int tile_j, int tiles,
+                            float *out, hipStream_t st);
+hipError_t launch_zero_counters(DevCounters *ctr, int full, hipStream_t st);
+hipError_t launch_combine_force(const double *acc, int split, int npad, int n, const double *m,
+                                double G, int force_mode, double *out, hipStream_t st);
+hipError_t launch_diag(const double *x, const double *y, const double *z, const double *vx,
+                       const double *vy, const double *vz, const double *m, const uint8_t *flags,
+                       int n, double G, double eps, double *scratch, int64_t scratch_len,
+                       hipStream_t st);
+hipError_t launch_render(const double *x, const double *y, const double *z, const uint8_t *flags,
+                         int n, float *xyz, hipStream_t st);
+
+}  // namespace nb

@@ -1,0 +1,188 @@
+"""Python driver of the C-ABI: body arrays and a context wrapper.
+
+`Bodies` is the SoA form of the reference's body array (Body fields X..Mass, CollisionBehavior,
+Exists/fragmenting/IsSun/Pinned, Body.r and FragFactor; cmd/body/body.go:34-52).  `NBodyGPU` owns
+one nb_ctx and exposes the entry points of include/nbody_gpu.h with numpy arrays.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib
+
+
+@dataclass
+class Bodies:
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    m: np.ndarray
+    r: np.ndarray
+    beh: np.ndarray = None
+    flags: np.ndarray = None
+    rest: np.ndarray = None
+    frag: np.ndarray = None
+
+    def __post_init__(self):
+        for k in ("x", "y", "z", "vx", "vy", "vz", "m", "r"):
+            setattr(self, k, np.ascontiguousarray(getattr(self, k), dtype=np.float64))
+        n = self.n
+        if self.beh is None:
+            self.beh = np.full(n, L.NB_CB_ELASTIC, np.int32)
+        if self.flags is None:
+            self.flags = np.full(n, L.NB_F_EXISTS, np.uint8)
+        self.beh = np.ascontiguousarray(self.beh, dtype=np.int32)
+        self.flags = np.ascontiguousarray(self.flags, dtype=np.uint8)
+        if self.rest is None:
+            self.rest = np.ones(n, np.float64)
+        self.rest = np.ascontiguousarray(self.rest, dtype=np.float64)
+        if self.frag is None:
+            self.frag = np.zeros(n, np.float64)
+        self.frag = np.ascontiguousarray(self.frag, dtype=np.float64)
+        for k in ("y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest", "frag"):
+            if getattr(self, k).shape != (n,):
+                raise ValueError("Bodies.%s has shape %s, expected (%d,)" %
+                                 (k, getattr(self, k).shape, n))
+
+    @property
+    def n(self):
+        return int(self.x.shape[0])
+
+    def copy(self):
+        return Bodies(*(getattr(self, k).copy() for k in
+                        ("x", "y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest",
+                         "frag")))
+
+    def pos(self):
+        return np.stack([self.x, self.y, self.z], axis=1)
+
+    def vel(self):
+        return np.stack([self.vx, self.vy, self.vz], axis=1)
+
+    def active(self):
+        return ((self.flags & L.NB_F_EXISTS) != 0) & ((self.flags & L.NB_F_FRAGMENTING) == 0)
+
+
+def _dp(a):
+    return a.ctypes.data_as(L._dp) if a is not None else None
+
+
+PRECISIONS = {"fp32": L.NB_PREC_FP32, "fp64": L.NB_PREC_FP64}
+
+
+class NBodyGPU:
+    """One device context (nb_ctx)."""
+
+    def __init__(self, G=6.673e-11, eps=0.0, overlap_cutoff=True, collisions=True,
+                 precision="fp32", device=0, pair_capacity=0, ipl=0, tile_j=0, jsplit=0,
+                 block=0, variant=0):
+        p = L.NbParams()
+        check(lib.nb_params_init(C.byref(p)))
+        p.G = float(G)
+        p.eps = float(eps)
+        p.overlap_cutoff = int(bool(overlap_cutoff))
+        p.collisions = int(bool(collisions))
+        p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        p.device = int(device)
+        p.pair_capacity = int(pair_capacity)
+        p.ipl, p.tile_j, p.jsplit, p.block, p.variant = ipl, tile_j, jsplit, block, variant
+        self.params = p
+        self._ctx = C.c_void_p()
+        check(lib.nb_create(C.byref(p), C.byref(self._ctx)), None)
+        self.n = 0
+
+    def close(self):
+        if self._ctx:
+            lib.nb_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        return check(rc, self._ctx)
+
+    def load(self, b: Bodies):
+        self._check(lib.nb_load(self._ctx, b.n, _dp(b.x), _dp(b.y), _dp(b.z), _dp(b.vx),
+                                _dp(b.vy), _dp(b.vz), _dp(b.m), _dp(b.r),
+                                b.beh.ctypes.data_as(L._i32p), b.flags.ctypes.data_as(L._u8p),
+                                _dp(b.rest), _dp(b.frag)))
+        self.n = b.n
+        self._template = b
+        return self
+
+    def step(self, dt, R=1.0, nsteps=1, allow_overflow=False):
+        st = L.NbStats()
+        rc = lib.nb_step(self._ctx, float(dt), float(R), int(nsteps), C.byref(st))
+        if not (allow_overflow and rc == L.NB_E_PAIR_OVERFLOW):
+            self._check(rc)
+        return st.as_dict()
+
+    def forces(self):
+        f = np.empty((3, self.n), np.float64)
+        self._check(lib.nb_compute_forces(self._ctx, _dp(f[0]), _dp(f[1]), _dp(f[2])))
+        return f.T.copy()
+
+    def accel_fp64(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        a = np.empty((3, idx.shape[0]), np.float64)
+        self._check(lib.nb_accel_fp64(self._ctx, idx.ctypes.data_as(L._i64p), idx.shape[0],
+                                      _dp(a[0]), _dp(a[1]), _dp(a[2])))
+        return a.T.copy()
+
+    def pairs(self):
+        cnt = C.c_int64()
+        self._check(lib.nb_get_pairs(self._ctx, None, 0, C.byref(cnt)))
+        k = cnt.value
+        out = np.empty((max(k, 1), 2), np.int32)
+        self._check(lib.nb_get_pairs(self._ctx, out.ctypes.data_as(L._i32p), k, C.byref(cnt)))
+        return out[:min(k, cnt.value)].copy()
+
+    def store(self):
+        n = self.n
+        b = self._template.copy()
+        self._check(lib.nb_store(self._ctx, _dp(b.x), _dp(b.y), _dp(b.z), _dp(b.vx), _dp(b.vy),
+                                 _dp(b.vz), b.flags.ctypes.data_as(L._u8p), _dp(b.rest)))
+        assert b.n == n
+        return b
+
+    def render(self):
+        xyz = np.empty((self.n, 3), np.float32)
+        ex = np.empty(self.n, np.uint8)
+        self._check(lib.nb_store_render(self._ctx, xyz.ctypes.data_as(L._fp),
+                                        ex.ctypes.data_as(L._u8p)))
+        return xyz, ex
+
+    def diagnostics(self):
+        d = L.NbDiag()
+        self._check(lib.nb_diagnostics(self._ctx, C.byref(d)))
+        return {"kinetic": d.kinetic, "potential": d.potential,
+                "momentum": np.array(list(d.momentum)), "mass": d.mass,
+                "n_active": d.n_active}
+
+    def set_timing(self, on=True):
+        self._check(lib.nb_set_timing(self._ctx, 1 if on else 0))
+
+    def config(self):
+        cfg = (C.c_int32 * 5)()
+        self._check(lib.nb_get_config(self._ctx, cfg))
+        return dict(zip(("ipl", "tile_j", "jsplit", "block", "variant"), list(cfg)))
+
+    def autotune(self, reps=3):
+        cfg = (C.c_int32 * 5)()
+        self._check(lib.nb_autotune(self._ctx, int(reps), cfg))
+        return dict(zip(("ipl", "tile_j", "jsplit", "block", "variant"), list(cfg)))

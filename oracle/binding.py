@@ -1,0 +1,202 @@
+"""ctypes binding of the C oracle (oracle/nb_oracle.h) -- TEST INFRASTRUCTURE ONLY.
+
+Works on plain numpy arrays (no dependency on the product package), so the oracle can check the
+GPU path without sharing code with it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_nb.so")
+
+F_EXISTS, F_FRAGMENTING, F_SUN, F_PINNED = 1, 2, 4, 8
+CB_NONE, CB_SUBSUME, CB_ELASTIC, CB_FRAGMENT = 0, 1, 2, 3
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+class OrState(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.POINTER(C.c_double)) for k in
+                                     ("x", "y", "z", "vx", "vy", "vz", "m", "r", "rest",
+                                      "frag")] + [
+        ("beh", C.POINTER(C.c_int32)), ("flags", C.POINTER(C.c_uint8)),
+        ("collided", C.POINTER(C.c_uint8))]
+
+
+class OrParams(C.Structure):
+    _fields_ = [("G", C.c_double), ("eps", C.c_double), ("cutoff", C.c_int32),
+                ("collisions", C.c_int32), ("nthreads", C.c_int32)]
+
+
+class OrStats(C.Structure):
+    _fields_ = [("pairs_total", C.c_int64), ("pairs_last", C.c_int64),
+                ("resolved_total", C.c_int64), ("fragment_requests", C.c_int64),
+                ("removed_total", C.c_int64)]
+
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        sp = C.POINTER(OrState)
+        L.or_force_ref.argtypes = [sp, C.c_double, C.POINTER(C.c_int64), C.c_int64, _dp, _dp, _dp,
+                                   C.c_int]
+        L.or_force_soft.argtypes = [sp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64),
+                                    C.c_int64, _dp, _dp, _dp, C.c_int]
+        L.or_force_soft_ld.argtypes = [sp, C.c_double, C.c_double, C.c_int,
+                                       C.POINTER(C.c_int64), C.c_int64, _dp, _dp, _dp]
+        L.or_detect.argtypes = [sp, C.POINTER(C.c_int32), C.c_int64]
+        L.or_detect.restype = C.c_int64
+        L.or_elastic.argtypes = [sp, C.c_int64, C.c_int64, _dp]
+        L.or_resolve.argtypes = [sp, C.POINTER(C.c_int32), C.c_int64, C.POINTER(OrStats)]
+        L.or_update.argtypes = [sp, _dp, _dp, _dp, C.c_double, C.c_double, C.POINTER(OrStats)]
+        L.or_step.argtypes = [sp, C.POINTER(OrParams), C.c_double, C.c_double, C.c_int,
+                              C.POINTER(C.c_int32), C.c_int64, C.POINTER(OrStats)]
+        L.or_energy.argtypes = [sp, C.c_double, C.c_double, _dp, C.c_int]
+        L.or_bench_workpool.argtypes = [sp, C.c_double, C.c_int64, C.c_int, C.c_int, _dp,
+                                        C.POINTER(C.c_int64)]
+        L.or_bench_workpool.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "m", "r")
+
+
+class State:
+    """Owns contiguous copies of a body set; .s is the or_state view."""
+
+    def __init__(self, x, y, z, vx, vy, vz, m, r, beh=None, flags=None, rest=None, frag=None):
+        n = len(x)
+        self.a = {}
+        for k, v in zip(FIELDS, (x, y, z, vx, vy, vz, m, r)):
+            self.a[k] = np.array(v, dtype=np.float64, copy=True)
+        self.a["rest"] = np.ones(n) if rest is None else np.array(rest, np.float64, copy=True)
+        self.a["frag"] = np.zeros(n) if frag is None else np.array(frag, np.float64, copy=True)
+        self.beh = (np.full(n, CB_ELASTIC, np.int32) if beh is None
+                    else np.array(beh, np.int32, copy=True))
+        self.flags = (np.full(n, F_EXISTS, np.uint8) if flags is None
+                      else np.array(flags, np.uint8, copy=True))
+        self.collided = np.zeros(n, np.uint8)
+        self.n = n
+        s = OrState()
+        s.n = n
+        for k in FIELDS + ("rest", "frag"):
+            setattr(s, k, self.a[k].ctypes.data_as(_dp))
+        s.beh = self.beh.ctypes.data_as(C.POINTER(C.c_int32))
+        s.flags = self.flags.ctypes.data_as(C.POINTER(C.c_uint8))
+        s.collided = self.collided.ctypes.data_as(C.POINTER(C.c_uint8))
+        self.s = s
+
+    @classmethod
+    def from_bodies(cls, b):
+        return cls(b.x, b.y, b.z, b.vx, b.vy, b.vz, b.m, b.r, b.beh, b.flags, b.rest, b.frag)
+
+    def __getattr__(self, k):
+        a = self.__dict__.get("a")
+        if a is not None and k in a:
+            return a[k]
+        raise AttributeError(k)
+
+    def pos(self):
+        return np.stack([self.a["x"], self.a["y"], self.a["z"]], axis=1)
+
+    def vel(self):
+        return np.stack([self.a["vx"], self.a["vy"], self.a["vz"]], axis=1)
+
+
+def _idx(idx):
+    if idx is None:
+        return None, 0, None
+    idx = np.ascontiguousarray(idx, np.int64)
+    return idx, idx.shape[0], idx.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def force_ref(st, G=6.673e-11, idx=None, nthreads=8):
+    idx, k, ip = _idx(idx)
+    k = st.n if idx is None else k
+    f = np.zeros((3, k))
+    lib().or_force_ref(C.byref(st.s), G, ip, k, *(f[c].ctypes.data_as(_dp) for c in range(3)),
+                       nthreads)
+    return f.T.copy()
+
+
+def force_soft(st, G, eps, cutoff=False, idx=None, nthreads=8, long_double=False):
+    idx, k, ip = _idx(idx)
+    k = st.n if idx is None else k
+    f = np.zeros((3, k))
+    if long_double:
+        lib().or_force_soft_ld(C.byref(st.s), G, eps, int(cutoff), ip, k,
+                               *(f[c].ctypes.data_as(_dp) for c in range(3)))
+    else:
+        lib().or_force_soft(C.byref(st.s), G, eps, int(cutoff), ip, k,
+                            *(f[c].ctypes.data_as(_dp) for c in range(3)), nthreads)
+    return f.T.copy()
+
+
+def detect(st):
+    n = lib().or_detect(C.byref(st.s), None, 0)
+    out = np.zeros((max(n, 1), 2), np.int32)
+    lib().or_detect(C.byref(st.s), out.ctypes.data_as(C.POINTER(C.c_int32)), n)
+    return out[:n]
+
+
+def elastic(st, i, j):
+    out = np.zeros(9)
+    ok = lib().or_elastic(C.byref(st.s), i, j, out.ctypes.data_as(_dp))
+    return bool(ok), out
+
+
+def resolve(st, pairs):
+    pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    stats = OrStats()
+    lib().or_resolve(C.byref(st.s), pairs.ctypes.data_as(C.POINTER(C.c_int32)), pairs.shape[0],
+                     C.byref(stats))
+    return {f[0]: getattr(stats, f[0]) for f in OrStats._fields_}
+
+
+def update(st, f, dt, R):
+    f = np.ascontiguousarray(np.asarray(f, np.float64).T)
+    stats = OrStats()
+    lib().or_update(C.byref(st.s), *(f[c].ctypes.data_as(_dp) for c in range(3)), dt, R,
+                    C.byref(stats))
+    return {k[0]: getattr(stats, k[0]) for k in OrStats._fields_}
+
+
+def step(st, G, eps, cutoff, collisions, dt, R, nsteps=1, nthreads=8):
+    p = OrParams(G, eps, int(cutoff), int(collisions), nthreads)
+    stats = OrStats()
+    cap = 1 << 20
+    pairs = np.zeros((cap, 2), np.int32)
+    rc = lib().or_step(C.byref(st.s), C.byref(p), dt, R, nsteps,
+                       pairs.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(stats))
+    if rc:
+        raise RuntimeError("or_step failed %d" % rc)
+    d = {k[0]: getattr(stats, k[0]) for k in OrStats._fields_}
+    d["pairs"] = pairs[:min(d["pairs_last"], cap)].copy()
+    return d
+
+
+def energy(st, G, eps, nthreads=8):
+    out = np.zeros(5)
+    lib().or_energy(C.byref(st.s), G, eps, out.ctypes.data_as(_dp), nthreads)
+    return {"kinetic": out[0], "potential": out[1], "momentum": out[2:5].copy()}
+
+
+def bench_workpool(st, G, ntargets, nsteps=1, nthreads=8, want_force=False):
+    f = np.zeros((ntargets, 3)) if want_force else None
+    hits = C.c_int64()
+    el = lib().or_bench_workpool(C.byref(st.s), G, ntargets, nsteps, nthreads,
+                                 f.ctypes.data_as(_dp) if want_force else None, C.byref(hits))
+    return el, hits.value, f
