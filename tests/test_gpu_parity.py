@@ -179,7 +179,10 @@ def test_boundary_pairs_exact(oracle):
     b.x[:6] = [0.0, 3.0, 100.0, 103.0, 200.0, 200.0]
     b.y[:6] = [0.0, 4.0, 0.0, 4.0, 0.0, 0.0]
     b.z[:6] = [2000.0] * 6
-    b.r[:6] = [2.5, 2.5, 2.5, np.nextafter(2.5, 0), 1.0, 1.0]
+    # 2.5 + nextafter(2.5, 0) rounds back to 5.0 (a tie, to even); two ulps below does not
+    r3 = np.nextafter(np.nextafter(2.5, 0), 0)
+    b.r[:6] = [2.5, 2.5, 2.5, r3, 1.0, 1.0]
+    assert 2.5 + r3 < 5.0
     st = oracle.State.from_bodies(b)
     for prec in ("fp32", "fp64"):
         with NBodyGPU(precision=prec) as g:
