@@ -43,7 +43,7 @@ static hipError_t dalloc(T **p, size_t count) {
 static void free_all(Ctx *c) {
   void *ptrs[] = {c->x,    c->y,      c->z,     c->vx,   c->vy,        c->vz,   c->m,
                   c->r,    c->rest,   c->frag,  c->beh,  c->flags,     c->collided,
-                  c->js,   c->rmax_tile, c->acc, c->fcnt, c->flist,    c->pairs, c->ctr,
+                  c->js,   c->rmax_tile, c->acc, c->fcnt, c->flist, c->recs, c->pairs, c->ctr,
                   c->diag};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -54,6 +54,7 @@ static void free_all(Ctx *c) {
   c->rmax_tile = nullptr;
   c->acc = nullptr;
   c->fcnt = c->flist = nullptr;
+  c->recs = nullptr;
   c->pairs = nullptr;
   c->ctr = nullptr;
   c->diag = nullptr;
@@ -101,6 +102,7 @@ static int prepare_shape(Ctx *c) {
     NB_HIP(c, dalloc(&c->acc, (size_t)alloc_split * 3 * c->cap));
     NB_HIP(c, dalloc(&c->fcnt, (size_t)alloc_split * c->cap));
     NB_HIP(c, dalloc(&c->flist, (size_t)alloc_split * c->cap * kFlagSlots));
+    NB_HIP(c, dalloc(&c->recs, (size_t)alloc_split * c->cap));
     c->acc_split = alloc_split;
   }
   const int tiles = tiles_of(c);
@@ -130,6 +132,9 @@ static ForceArgs force_args(Ctx *c) {
   a.eps2 = (float)e2;
   a.detect = detect_needed(c) ? 1 : 0;
   a.drop_flagged = drop_needed(c) ? 1 : 0;
+  a.recs = c->recs;
+  a.rec_cap = (int64_t)c->acc_split * c->cap;
+  a.ctr = c->ctr;
   return a;
 }
 
@@ -167,6 +172,8 @@ static int force_phase(Ctx *c, bool timed, int ev_slot) {
       f.acc = c->acc;
       f.fcnt = c->fcnt;
       f.flist = c->flist;
+      f.recs = c->recs;
+      f.rec_cap = a.rec_cap;
       f.pairs = c->pairs;
       f.ctr = c->ctr;
       f.pair_cap = c->pair_cap;

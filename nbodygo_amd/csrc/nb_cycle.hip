@@ -483,6 +483,7 @@ __global__ void k_zero_counters(DevCounters *c, int full) {
     *c = z;
   } else {
     c->pair_count = 0;
+    c->rec_count = 0;
   }
 }
 

@@ -84,23 +84,58 @@ __device__ __forceinline__ void tile_end(TargetState<IPL> &T, const float (&px)[
   }
 }
 
+// One source against the lane's IPL targets, written phase by phase across the targets so that
+// each v_rsq_f32 result is consumed several instructions later (no transcendental-use stall).
 #define NB_INTERACT(bx, by, bz, bw)                                   \
-  _Pragma("unroll") for (int k = 0; k < IPL; k++) {                    \
-    float dx = (bx) - T.xi[k];                                         \
-    float dy = (by) - T.yi[k];                                         \
-    float dz = (bz) - T.zi[k];                                         \
-    float r2 = __builtin_fmaf(dx, dx, eps2);                           \
-    r2 = __builtin_fmaf(dy, dy, r2);                                   \
-    r2 = __builtin_fmaf(dz, dz, r2);                                   \
-    float inv = __builtin_amdgcn_rsqf(r2);                             \
-    float s = (bw) * inv;                                              \
-    float inv2 = inv * inv;                                            \
-    s = s * inv2;                                                      \
-    px[k] = __builtin_fmaf(dx, s, px[k]);                              \
-    py[k] = __builtin_fmaf(dy, s, py[k]);                              \
-    pz[k] = __builtin_fmaf(dz, s, pz[k]);                              \
-    if (DETECT) rmin[k] = __builtin_fminf(rmin[k], r2);                \
+  {                                                                    \
+    float dx[IPL], dy[IPL], dz[IPL], r2[IPL], inv[IPL];                \
+    _Pragma("unroll") for (int k = 0; k < IPL; k++) {                  \
+      dx[k] = (bx) - T.xi[k];                                          \
+      dy[k] = (by) - T.yi[k];                                          \
+      dz[k] = (bz) - T.zi[k];                                          \
+      r2[k] = __builtin_fmaf(dx[k], dx[k], eps2);                      \
+      r2[k] = __builtin_fmaf(dy[k], dy[k], r2[k]);                     \
+      r2[k] = __builtin_fmaf(dz[k], dz[k], r2[k]);                     \
+      inv[k] = __builtin_amdgcn_rsqf(r2[k]);                           \
+    }                                                                  \
+    _Pragma("unroll") for (int k = 0; k < IPL; k++) {                  \
+      float s = (bw) * inv[k];                                         \
+      float inv2 = inv[k] * inv[k];                                    \
+      s = s * inv2;                                                    \
+      px[k] = __builtin_fmaf(dx[k], s, px[k]);                         \
+      py[k] = __builtin_fmaf(dy[k], s, py[k]);                         \
+      pz[k] = __builtin_fmaf(dz[k], s, pz[k]);                         \
+      if (DETECT) rmin[k] = __builtin_fminf(rmin[k], r2[k]);           \
+    }                                                                  \
   }
+
+// With eps > 0 the self pair contributes exactly zero force but its r^2 = eps^2 would flag every
+// target's own tile.  For that one tile the minimum is recomputed without the self pair (same
+// FP32 operations, so the other pairs' r^2 are bit-identical).  A wave's targets are 64
+// consecutive bodies, so the branch is wave-uniform for tile_j >= 64.
+template <int IPL, bool DETECT, class Src>
+__device__ __forceinline__ void self_tile_rmin(const TargetState<IPL> &T, float (&rmin)[IPL],
+                                               int base, int stride, int t, int tile_j,
+                                               float eps2, Src src) {
+  if (!DETECT || !(eps2 > 0.f)) return;
+#pragma unroll
+  for (int k = 0; k < IPL; k++) {
+    const int sl = base + k * stride - t * tile_j;
+    if (sl >= 0 && sl < tile_j) {
+      float mn = FLT_MAX;
+      for (int jj = 0; jj < tile_j; jj++) {
+        if (jj == sl) continue;
+        const f4v b = src(jj);
+        const float dx = b.x - T.xi[k], dy = b.y - T.yi[k], dz = b.z - T.zi[k];
+        float r2 = __builtin_fmaf(dx, dx, eps2);
+        r2 = __builtin_fmaf(dy, dy, r2);
+        r2 = __builtin_fmaf(dz, dz, r2);
+        mn = __builtin_fminf(mn, r2);
+      }
+      rmin[k] = mn;
+    }
+  }
+}
 
 template <int IPL, bool DETECT>
 __device__ __forceinline__ void load_targets(TargetState<IPL> &T, const ForceArgs &a, int base,
@@ -139,6 +174,10 @@ __device__ __forceinline__ void store_targets(const TargetState<IPL> &T, const F
       a.fcnt[q] = T.fc[k];
       int4 l = make_int4(T.f0[k], T.f1[k], T.f2[k], T.f3[k]);
       reinterpret_cast<int4 *>(a.flist)[q] = l;
+      if (T.fc[k] > 0 && i < a.n) {
+        const uint32_t r = atomicAdd(&a.ctr->rec_count, 1u);
+        if ((int64_t)r < a.rec_cap) a.recs[r] = make_int2(slice, i);
+      }
     }
   }
 }
@@ -167,6 +206,8 @@ __global__ __launch_bounds__(256) void k_force_scalar(ForceArgs a) {
       const f4v b = jt[jj];
       NB_INTERACT(b.x, b.y, b.z, b.w)
     }
+    self_tile_rmin<IPL, DETECT>(T, rmin, base, stride, t, a.tile_j, eps2,
+                                [&](int jj) -> f4v { return jt[jj]; });
     tile_end<IPL, DETECT>(T, px, py, pz, rmin, DETECT ? rmt[t] : 0.f, eps2, a.drop_flagged, t);
   }
   store_targets<IPL, DETECT>(T, a, base, stride);
@@ -199,6 +240,10 @@ __global__ __launch_bounds__(256) void k_force_lds(ForceArgs a) {
       const float4 b = sh[jj];
       NB_INTERACT(b.x, b.y, b.z, b.w)
     }
+    self_tile_rmin<IPL, DETECT>(T, rmin, base, stride, t, a.tile_j, eps2, [&](int jj) -> f4v {
+      const float4 b = sh[jj];
+      return f4v{b.x, b.y, b.z, b.w};
+    });
     tile_end<IPL, DETECT>(T, px, py, pz, rmin, DETECT ? rmt[t] : 0.f, eps2, a.drop_flagged, t);
   }
   store_targets<IPL, DETECT>(T, a, base, stride);
@@ -248,67 +293,84 @@ __device__ void emit_pair(uint64_t *pairs, DevCounters *ctr, int64_t cap, int i,
     atomicOr(&ctr->pair_overflow, 1u);
 }
 
+// One wavefront per record: the 64 lanes split each flagged tile's sources, and the FP64 partial
+// sums are combined with a fixed butterfly, so the result does not depend on scheduling.  The
+// grid is fixed; waves stride over the record count the force kernel left in device memory.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_fixup(FixupArgs a) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (int64_t)a.split * a.npad) return;
-  const int slice = (int)(gid / a.npad);
-  const int i = (int)(gid % a.npad);
-  if (i >= a.n) return;
-  const size_t q = (size_t)slice * a.npad + i;
-  const int cnt = a.fcnt[q];
-  if (cnt == 0 || !active_body(a.flags[i])) return;
-  const int4 lst = reinterpret_cast<const int4 *>(a.flist)[q];
-  const int t0 = slice * a.tiles_per_slice;
-  const int t1 = min(t0 + a.tiles_per_slice, a.tiles);
-  const bool all = cnt > kFlagSlots;
-  const int ntile = all ? (t1 - t0) : cnt;
-  const double xi = a.x[i], yi = a.y[i], zi = a.z[i], ri = a.r[i];
-  const bool ci = can_collide(a.beh[i]);
-  double ax = 0, ay = 0, az = 0;
-  for (int u = 0; u < ntile; u++) {
-    int t;
-    if (all) t = t0 + u;
-    else t = u == 0 ? lst.x : u == 1 ? lst.y : u == 2 ? lst.z : lst.w;
-    const int j0 = t * a.tile_j;
-    const int j1 = min(j0 + a.tile_j, a.n);
-    for (int j = j0; j < j1; j++) {
-      if (j == i || !active_body(a.flags[j])) continue;
-      const double dx = a.x[j] - xi;
-      const double dy = a.y[j] - yi;
-      const double dz = a.z[j] - zi;
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      const bool overlap = !(__builtin_sqrt(d2) > ri + a.r[j]);
-      if (overlap && a.collisions && i < j && ci && can_collide(a.beh[j]))
-        emit_pair(a.pairs, a.ctr, a.pair_cap, i, j);
-      if (a.recompute && !(overlap && a.cutoff)) {
-        const double ds = d2 + a.eps2;
-        const double w = a.m[j] / (ds * __builtin_sqrt(ds));
-        ax += w * dx;
-        ay += w * dy;
-        az += w * dz;
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+  const int nrec = (int)min((int64_t)a.ctr->rec_count, a.rec_cap);
+  for (int rr = wave; rr < nrec; rr += nwaves) {
+    const int2 rec = a.recs[rr];
+    const int slice = rec.x, i = rec.y;
+    if (!active_body(a.flags[i])) continue;
+    const size_t q = (size_t)slice * a.npad + i;
+    const int cnt = a.fcnt[q];
+    const int4 lst = reinterpret_cast<const int4 *>(a.flist)[q];
+    const int t0 = slice * a.tiles_per_slice;
+    const int t1 = min(t0 + a.tiles_per_slice, a.tiles);
+    const bool all = cnt > kFlagSlots;
+    const int ntile = all ? (t1 - t0) : cnt;
+    const double xi = a.x[i], yi = a.y[i], zi = a.z[i], ri = a.r[i];
+    const bool ci = can_collide(a.beh[i]);
+    double ax = 0, ay = 0, az = 0;
+    for (int u = 0; u < ntile; u++) {
+      int t;
+      if (all) t = t0 + u;
+      else t = u == 0 ? lst.x : u == 1 ? lst.y : u == 2 ? lst.z : lst.w;
+      const int j0 = t * a.tile_j;
+      const int j1 = min(j0 + a.tile_j, a.n);
+      for (int j = j0 + lane; j < j1; j += 64) {
+        if (j == i || !active_body(a.flags[j])) continue;
+        const double dx = a.x[j] - xi;
+        const double dy = a.y[j] - yi;
+        const double dz = a.z[j] - zi;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const bool overlap = !(__builtin_sqrt(d2) > ri + a.r[j]);
+        if (overlap && a.collisions && i < j && ci && can_collide(a.beh[j]))
+          emit_pair(a.pairs, a.ctr, a.pair_cap, i, j);
+        if (a.recompute && !(overlap && a.cutoff)) {
+          const double ds = d2 + a.eps2;
+          const double w = a.m[j] / (ds * __builtin_sqrt(ds));
+          ax += w * dx;
+          ay += w * dy;
+          az += w * dz;
+        }
       }
     }
-  }
-  atomicAdd((unsigned long long *)&a.ctr->fixup_tiles, (unsigned long long)ntile);
-  if (a.recompute) {
-    double *o = a.acc + (size_t)slice * 3 * a.npad;
-    if (all) {
-      o[i] = ax;
-      o[a.npad + i] = ay;
-      o[2 * a.npad + i] = az;
-    } else {
-      o[i] += ax;
-      o[a.npad + i] += ay;
-      o[2 * a.npad + i] += az;
+    if (a.recompute) {
+      ax = wave_sum(ax);
+      ay = wave_sum(ay);
+      az = wave_sum(az);
+    }
+    if (lane == 0) {
+      atomicAdd((unsigned long long *)&a.ctr->fixup_tiles, (unsigned long long)ntile);
+      if (a.recompute) {
+        double *o = a.acc + (size_t)slice * 3 * a.npad;
+        if (all) {
+          o[i] = ax;
+          o[a.npad + i] = ay;
+          o[2 * a.npad + i] = az;
+        } else {
+          o[i] += ax;
+          o[a.npad + i] += ay;
+          o[2 * a.npad + i] += az;
+        }
+      }
     }
   }
 }
 
 hipError_t launch_fixup(const FixupArgs &a, hipStream_t st) {
-  const int64_t total = (int64_t)a.split * a.npad;
-  const int block = 256;
-  hipLaunchKernelGGL(k_fixup, dim3((unsigned)((total + block - 1) / block)), dim3(block), 0, st,
-                     a);
+  // 8 waves per CU on 256 CUs; the kernel strides over however many records there are.
+  hipLaunchKernelGGL(k_fixup, dim3(512), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ constexpr float kSentinel = 1.0e18f; // position of padding / inactive sources i
 // step sequence stays capturable).
 struct DevCounters {
   uint32_t pair_count;       // pairs appended in the current force phase
-  uint32_t fixup_count;      // unused slot kept for alignment
+  uint32_t rec_count;        // (slice, target) fix-up records appended in the current phase
   uint64_t pairs_total;
   uint64_t pairs_last;
   uint64_t resolved_total;
@@ -68,8 +68,9 @@ struct Ctx {
   // per-slice accelerations (FP32 path) or force (FP64 path): [split][3][npad] doubles
   double *acc = nullptr;
   int acc_split = 0;
-  // flagged tiles: counts [split][npad], lists [split][npad][kFlagSlots]
+  // flagged tiles: counts [split][npad], lists [split][npad][kFlagSlots], records [split*npad]
   int32_t *fcnt = nullptr, *flist = nullptr;
+  int2 *recs = nullptr;
 
   // collision pairs as 64-bit keys (i << 32 | j)
   uint64_t *pairs = nullptr;
@@ -110,6 +111,9 @@ struct ForceArgs {
   float eps2;
   int detect;           // track per-tile minimum distance and flag tiles
   int drop_flagged;     // flagged tiles' FP32 partials are discarded (recomputed in FP64)
+  int2 *recs;           // (slice, target) of every target with flagged tiles
+  int64_t rec_cap;      // entries in recs (slices * padded bodies)
+  DevCounters *ctr;
 };
 hipError_t launch_force_f32(const ForceShape &s, const ForceArgs &a, hipStream_t st);
 
@@ -120,6 +124,8 @@ struct FixupArgs {
   double *acc;
   const int32_t *fcnt;
   const int32_t *flist;
+  const int2 *recs;
+  int64_t rec_cap;
   uint64_t *pairs;
   DevCounters *ctr;
   int64_t pair_cap;
