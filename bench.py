@@ -29,19 +29,66 @@ WORKLOADS = {
                                      "radius 1e-4"),
     "plummer262k": dict(n=262144, desc="Plummer sphere N=262144 (a=1, M=1, G=1), eps=0.01, "
                                        "dt=1e-3, FP32 force + Kahan, FP64 update, collisions off"),
+    "batch512": dict(n=0, desc="512 independent simulations, N uniform in [256, 8192] (seeded), "
+                               "uniform-sphere clusters with lognormal masses, eps=0.01, "
+                               "dt=1e-3, FP32 force + Kahan, FP64 update, one hipGraph launch "
+                               "per call, autotuned target-block widths"),
 }
+
+
+def run_batch(a, w):
+    """Config 3: the batched path; a step advances all 512 simulations by one cycle."""
+    from nbodygo_amd import NBodyBatch, gen
+    sizes = gen.batch_sizes(512, seed=2024)
+    sims = gen.batch_clusters(sizes, seed=2024)
+    bt = NBodyBatch(G=1.0, eps=0.01)
+    bt.load(sims)
+    shapes = bt.shapes() if a.no_tune else bt.autotune(reps=2)
+    if a.warmup:
+        bt.step(1e-3, 1.0, nsteps=a.warmup)
+    t0 = time.perf_counter()
+    s = bt.step(1e-3, 1.0, nsteps=a.steps)
+    el = time.perf_counter() - t0
+    per_step = s["interactions"] / a.steps
+    ms_step = el / a.steps * 1e3
+    achieved = FLOPS_PER_INTERACTION * per_step / (ms_step * 1e-3)
+    out = {
+        "metric": METRIC, "value": s["interactions"] / el / 1e9, "unit": "G interactions/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
+        "steps_per_s": 1e3 / ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded clusters)",
+        "config": {"workload": w["desc"], "sims": 512, "bodies": int(sizes.sum()),
+                   "ipl_per_bucket": shapes},
+        "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12,
+                     "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": None, "kernel": "k_batch_force (whole-step time, update included)",
+                     "flops_per_interaction": FLOPS_PER_INTERACTION,
+                     "interactions_per_launch": per_step},
+    }
+    bt.close()
+    print(json.dumps(out))
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="plummer65k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default kernel shape")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--shape", default="",
+                    help="pin the FP32 force kernel shape: ipl,tile_j,jsplit,block,variant")
+    ap.add_argument("--no-elastic", action="store_true", help="skip the elastic-collision run")
     return ap.parse_args()
+
+
+def shape_kw(spec):
+    if not spec:
+        return {}
+    v = [int(q) for q in spec.split(",")]
+    return dict(zip(("ipl", "tile_j", "jsplit", "block", "variant"), v))
 
 
 def force_error(g, b, nsample=4096, seed=0):
@@ -99,11 +146,14 @@ def main():
     from nbodygo_amd import NBodyGPU, gen
 
     w = WORKLOADS[a.workload]
+    if a.workload == "batch512":
+        return run_batch(a, w)
     n = w["n"]
     b = gen.plummer(n, seed=2024)
-    g = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=False)
+    pinned = shape_kw(a.shape)
+    g = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=False, **pinned)
     g.load(b)
-    shape = g.config() if a.no_tune else g.autotune(reps=3)
+    shape = g.config() if (a.no_tune or pinned) else g.autotune(reps=3)
     s, el = timed_run(g, b, a.steps, a.warmup)
     inter = s["interactions"]
     ms_step = el / a.steps * 1e3
@@ -134,11 +184,11 @@ def main():
                      "interactions_per_launch": per_launch, "avg_launch_ms": launch_ms},
         "force_error": err,
     }
-    if a.workload == "plummer65k":
-        ge = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=True, collisions=True)
+    if a.workload == "plummer65k" and not a.no_elastic:
+        ge = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=True, collisions=True, **pinned)
         be = gen.plummer(n, seed=2024, radius=1e-4)
         ge.load(be)
-        if not a.no_tune:
+        if not (a.no_tune or pinned):
             ge.autotune(reps=3)
         se, ele = timed_run(ge, be, a.steps, a.warmup)
         out["elastic"] = {"value": se["interactions"] / ele / 1e9, "unit": "G interactions/s",

@@ -169,6 +169,65 @@ int nb_get_config(nb_ctx *ctx, int32_t *cfg5);
  * cfg5 (may be NULL).  reps = timed launches per candidate. */
 int nb_autotune(nb_ctx *ctx, int32_t reps, int32_t *cfg5);
 
+/* ---- batched path: many independent simulations in one persistent launch ------------------ */
+/* No counterpart in the reference (one BodyCollection per process); for callers that run many
+ * small simulations (parameter sweeps, ensembles).  Each simulation follows the per-cycle
+ * semantics above with the softened FP32 force (eps > 0) and FP64 update; collisions and the
+ * overlap cutoff are not part of this path (nb_batch_create returns NB_E_UNSUPPORTED). */
+typedef struct nb_batch nb_batch;
+
+int nb_batch_create(const nb_params *p, nb_batch **out);
+void nb_batch_destroy(nb_batch *b);
+const char *nb_batch_last_error(const nb_batch *b);
+
+/* Simulation k owns bodies [offsets[k], offsets[k+1]); offsets has nsims + 1 entries, offsets[0]
+ * = 0.  flags may be NULL (all existing). */
+int nb_batch_load(nb_batch *b, int32_t nsims, const int64_t *offsets, const double *x,
+                  const double *y, const double *z, const double *vx, const double *vy,
+                  const double *vz, const double *m, const uint8_t *flags);
+
+/* nsteps cycles of every simulation as one hipGraph launch (captured on first use for a given
+ * (nsteps, dt, R), replayed afterwards). */
+int nb_batch_step(nb_batch *b, double dt, double R, int32_t nsteps, nb_stats *out);
+
+/* 1 = hipGraph launch (default), 0 = eager kernel launches (same kernels). */
+int nb_batch_set_graph(nb_batch *b, int32_t enable);
+
+/* Force on every body (unpadded body order), for tests. */
+int nb_batch_forces(nb_batch *b, double *fx, double *fy, double *fz);
+
+int nb_batch_store(nb_batch *b, double *x, double *y, double *z, double *vx, double *vy,
+                   double *vz, uint8_t *flags);
+
+/* Target-block width (targets per lane: 1, 2 or 4) for the size buckets N < 1024, < 4096,
+ * >= 4096.  nb_batch_autotune times each choice per bucket on the loaded simulations and keeps
+ * the fastest. */
+int nb_batch_get_shapes(nb_batch *b, int32_t *ipl3);
+int nb_batch_set_shapes(nb_batch *b, const int32_t *ipl3);
+int nb_batch_autotune(nb_batch *b, int32_t reps, int32_t *ipl3);
+
+/* ---- small MLP on the matrix cores (the learned part of the far-field surrogate) ----------- */
+/* No counterpart in the reference.  d_in -> hidden (x layers, GELU) -> d_out; BF16 weights and
+ * activations on MFMA with FP32 accumulation; FP32 master weights and Adam state.  Parameters
+ * are exchanged in the unpadded layout: per linear layer, W [in x out] row-major, then b [out]. */
+typedef struct nb_mlp nb_mlp;
+
+int nb_mlp_create(int32_t device, int32_t d_in, int32_t hidden, int32_t layers, int32_t d_out,
+                  uint64_t seed, nb_mlp **out);
+void nb_mlp_destroy(nb_mlp *m);
+const char *nb_mlp_last_error(const nb_mlp *m);
+int nb_mlp_param_count(const nb_mlp *m, int64_t *count);
+int nb_mlp_set_params(nb_mlp *m, const float *params, int64_t count);
+int nb_mlp_get_params(nb_mlp *m, float *params, int64_t count);
+/* y [rows x d_out] = MLP(x [rows x d_in]) */
+int nb_mlp_forward(nb_mlp *m, const float *x, int64_t rows, float *y);
+/* Gradient of the mean squared error over rows (a multiple of 64) w.r.t. the parameters. */
+int nb_mlp_grad(nb_mlp *m, const float *x, const float *t, int64_t rows, float *grad,
+                float *loss);
+/* steps Adam updates on consecutive minibatches of batch rows (a multiple of 64). */
+int nb_mlp_train(nb_mlp *m, const float *x, const float *t, int64_t rows, int32_t steps,
+                 int32_t batch, float lr, float *loss_out);
+
 #ifdef __cplusplus
 }
 #endif

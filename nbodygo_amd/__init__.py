@@ -13,6 +13,6 @@ Importing this package raises if libnbgpu.so has not been built: there is no CPU
 """
 from . import _lib
 from ._lib import NBodyError
-from .sim import Bodies, NBodyGPU
+from .sim import Bodies, NBodyBatch, NBodyGPU
 
-__all__ = ["Bodies", "NBodyGPU", "NBodyError", "_lib"]
+__all__ = ["Bodies", "NBodyBatch", "NBodyGPU", "NBodyError", "_lib"]

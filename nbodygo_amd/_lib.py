@@ -107,6 +107,18 @@ SIGNATURES = [
     ("nb_set_timing", C.c_int, [_ctx, C.c_int32]),
     ("nb_get_config", C.c_int, [_ctx, _i32p]),
     ("nb_autotune", C.c_int, [_ctx, C.c_int32, _i32p]),
+    ("nb_batch_create", C.c_int, [C.POINTER(NbParams), C.POINTER(_ctx)]),
+    ("nb_batch_destroy", None, [_ctx]),
+    ("nb_batch_last_error", C.c_char_p, [_ctx]),
+    ("nb_batch_load", C.c_int,
+     [_ctx, C.c_int32, _i64p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _u8p]),
+    ("nb_batch_step", C.c_int, [_ctx, C.c_double, C.c_double, C.c_int32, C.POINTER(NbStats)]),
+    ("nb_batch_set_graph", C.c_int, [_ctx, C.c_int32]),
+    ("nb_batch_forces", C.c_int, [_ctx, _dp, _dp, _dp]),
+    ("nb_batch_store", C.c_int, [_ctx, _dp, _dp, _dp, _dp, _dp, _dp, _u8p]),
+    ("nb_batch_get_shapes", C.c_int, [_ctx, _i32p]),
+    ("nb_batch_set_shapes", C.c_int, [_ctx, _i32p]),
+    ("nb_batch_autotune", C.c_int, [_ctx, C.c_int32, _i32p]),
 ]
 
 

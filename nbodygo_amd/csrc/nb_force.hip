@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_force_scalar(ForceArgs a) {
   const int stride = blockDim.x;
   const int base = blockIdx.x * blockDim.x * IPL + threadIdx.x;
   load_targets<IPL, DETECT>(T, a, base, stride);
-  const float eps2 = a.eps2;
+  const float eps2 = to_vgpr(a.eps2);
   const int t0 = blockIdx.y * a.tiles_per_slice;
   const int t1 = min(t0 + a.tiles_per_slice, a.tiles);
   cf4p js = (cf4p)a.js;
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_force_lds(ForceArgs a) {
   const int stride = blockDim.x;
   const int base = blockIdx.x * blockDim.x * IPL + threadIdx.x;
   load_targets<IPL, DETECT>(T, a, base, stride);
-  const float eps2 = a.eps2;
+  const float eps2 = to_vgpr(a.eps2);
   const int t0 = blockIdx.y * a.tiles_per_slice;
   const int t1 = min(t0 + a.tiles_per_slice, a.tiles);
   cfp rmt = (cfp)a.rmax_tile;

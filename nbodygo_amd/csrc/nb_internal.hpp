@@ -95,6 +95,15 @@ bool hip_ok(Ctx *c, hipError_t e, const char *what);
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// A wave-uniform value copied into a VGPR.  On gfx950 a VALU op whose operand is an SGPR does not
+// dual-issue (measured: ~61 vs ~90-110 FP32 lane-ops/clk/CU), so loop-invariant scalars used in
+// the pairwise inner loops are moved to VGPRs once.
+__device__ __forceinline__ float to_vgpr(float s) {
+  float v;
+  asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+  return v;
+}
+
 // ---- kernels launched from nb_api.cpp (defined in the .hip files) ----------------------------
 struct ForceArgs {
   const float4 *js;

@@ -1,0 +1,705 @@
+// nb_mlp.hip -- small MLP (d_in -> H -> ... -> H -> d_out, GELU) on gfx950 matrix cores.
+//
+// No counterpart in the reference.  It is the learned part of the far-field surrogate: BF16
+// weights and activations feed v_mfma_f32_16x16x16_bf16 with FP32 accumulation; master weights,
+// biases, pre-activations, gradients and the Adam state are FP32.
+//
+// GEMM: C[M x N] (FP32, row-major) = op(A)[M x K] * op(B)[K x N], A and B BF16 row-major, op =
+// identity or transpose.  A 256-thread workgroup computes a 64 x 64 tile as 2 x 2 waves of 32 x 32,
+// each wave 2 x 2 MFMA blocks of 16 x 16, stepping K by 16 through LDS.  Operand layout of the 16
+// x 16 x 16 BF16 MFMA (checked against a host GEMM in tests): lane l holds A[l % 16][4*(l/16) ..
+// +3] and B[4*(l/16) .. +3][l % 16]; the accumulator holds D[4*(l/16) + v][l % 16] in register v.
+// Split-K over grid.z writes partial tiles that a second kernel sums in fixed order.
+// Dimensions are padded by the caller: M, N multiples of 64, K a multiple of 16.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "nb_internal.hpp"
+
+namespace nb {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  // round to nearest even (inputs are finite)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
+__device__ __forceinline__ float gelu(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  return 0.5f * x * (1.f + tanhf(k0 * (x + k1 * x * x * x)));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  const float u = k0 * (x + k1 * x * x * x);
+  const float t = tanhf(u);
+  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k0 * (1.f + 3.f * k1 * x * x);
+}
+
+enum { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2 };
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm_bf16(const uint16_t *__restrict__ A,
+                                                   const uint16_t *__restrict__ B, float *C,
+                                                   int M, int N, int K, int k_per_split,
+                                                   int lda, int ldb, int epi,
+                                                   const float *__restrict__ bias,
+                                                   uint16_t *__restrict__ Hout) {
+  __shared__ uint16_t As[64][16 + 4];   // [m][k]
+  __shared__ uint16_t Bs[64][16 + 4];   // [n][k] (B tile stored transposed)
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int kb = blockIdx.z * k_per_split;
+  const int ke = min(K, kb + k_per_split);
+  f32x4 acc[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+    // A tile: op(A)[m0 .. m0+63][k0 .. k0+15]
+    if (!TA) {
+      const int r = tid >> 2, kk = (tid & 3) * 4;
+      const uint16_t *p = A + (size_t)(m0 + r) * lda + k0 + kk;
+      for (int e = 0; e < 4; e++) As[r][kk + e] = p[e];
+    } else {
+      const int kk = tid >> 4, mm = (tid & 15) * 4;  // A stored [K][M]
+      const uint16_t *p = A + (size_t)(k0 + kk) * lda + m0 + mm;
+      for (int e = 0; e < 4; e++) As[mm + e][kk] = p[e];
+    }
+    // B tile: op(B)[k0 .. k0+15][n0 .. n0+63] -> Bs[n][k]
+    if (!TB) {
+      const int kk = tid >> 4, nn = (tid & 15) * 4;  // B stored [K][N]
+      const uint16_t *p = B + (size_t)(k0 + kk) * ldb + n0 + nn;
+      for (int e = 0; e < 4; e++) Bs[nn + e][kk] = p[e];
+    } else {
+      const int nn = tid >> 2, kk = (tid & 3) * 4;   // B stored [N][K]
+      const uint16_t *p = B + (size_t)(n0 + nn) * ldb + k0 + kk;
+      for (int e = 0; e < 4; e++) Bs[nn][kk + e] = p[e];
+    }
+    __syncthreads();
+    const int kq = 4 * (lane >> 4);
+    for (int a = 0; a < 2; a++) {
+      const int row = wm * 32 + a * 16 + (lane & 15);
+      bf16x4 fa;
+      for (int e = 0; e < 4; e++) fa[e] = (short)As[row][kq + e];
+      for (int b = 0; b < 2; b++) {
+        const int col = wn * 32 + b * 16 + (lane & 15);
+        bf16x4 fb;
+        for (int e = 0; e < 4; e++) fb[e] = (short)Bs[col][kq + e];
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, acc[a][b], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  float *Cz = C + (size_t)blockIdx.z * M * N;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int v = 0; v < 4; v++) {
+        const int row = m0 + wm * 32 + a * 16 + 4 * (lane >> 4) + v;
+        const int col = n0 + wn * 32 + b * 16 + (lane & 15);
+        float val = acc[a][b][v];
+        if (epi == EPI_STORE) {
+          Cz[(size_t)row * N + col] = val;
+        } else {
+          val += bias[col];
+          Cz[(size_t)row * N + col] = val;  // pre-activation (or output)
+          if (epi == EPI_BIAS_GELU) Hout[(size_t)row * N + col] = f2bf(gelu(val));
+        }
+      }
+}
+
+// Sum split-K partials in fixed order: C[i] = sum_z P[z][i].
+__global__ void k_splitk_sum(const float *P, float *C, int64_t count, int splits) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; z++) s += P[(size_t)z * count + i];
+  C[i] = s;
+}
+
+__global__ void k_f32_to_bf16(const float *x, uint16_t *y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// Rows of width `cols` (row stride ld) converted and zero-padded to width ldp.
+__global__ void k_pad_rows_bf16(const float *x, int rows, int cols, int ld, uint16_t *y, int ldp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rows * ldp) return;
+  const int r = (int)(i / ldp), c = (int)(i % ldp);
+  y[i] = c < cols ? f2bf(x[(size_t)r * ld + c]) : 0;
+}
+
+// dZ = dH * gelu'(Z)  (BF16 out for the next GEMMs), keeping FP32 dZ for the bias gradient.
+__global__ void k_gelu_back(const float *dH, const float *Z, float *dZ, uint16_t *dZb, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = dH[i] * gelu_grad(Z[i]);
+  dZ[i] = g;
+  dZb[i] = f2bf(g);
+}
+
+// Column sums of X[rows][cols] (bias gradients): one thread per column, rows in order.
+__global__ void k_colsum(const float *X, int rows, int cols, float *out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; r++) s += X[(size_t)r * cols + c];
+  out[c] = s;
+}
+
+// MSE over the first dout columns: L = sum (y - t)^2 / (rows * dout); dY = 2 (y - t) / (rows *
+// dout) in FP32 and BF16; padded columns get zero gradient.  Per-row losses to lrow.
+__global__ void k_mse(const float *Y, int ldy, const float *T, int rows, int dout, float *dY,
+                      uint16_t *dYb, float *lrow) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float l = 0.f;
+  const float sc = 2.f / ((float)rows * (float)dout);
+  for (int c = 0; c < ldy; c++) {
+    float g = 0.f;
+    if (c < dout) {
+      const float d = Y[(size_t)r * ldy + c] - T[(size_t)r * dout + c];
+      l += d * d;
+      g = sc * d;
+    }
+    dY[(size_t)r * ldy + c] = g;
+    dYb[(size_t)r * ldy + c] = f2bf(g);
+  }
+  lrow[r] = l;
+}
+
+__global__ void k_adam(float *w, const float *g, float *m1, float *m2, uint16_t *wb, int64_t n,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gi;
+  const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  const float wi = w[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+}  // namespace nb
+
+// ---------------------------------------------------------------------------------------------
+// Host side: the MLP object (also used by the surrogate).
+// ---------------------------------------------------------------------------------------------
+using namespace nb;
+
+struct nb_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int din = 0, dinp = 0, hid = 0, layers = 0, dout = 0, doutp = 0;
+  // per linear layer l (0..layers): W_l [in_l x out_l] (FP32 master + BF16), b_l [out_l]
+  std::vector<int> in_dim, out_dim;           // padded
+  std::vector<float *> W, Wg, Wm1, Wm2, bvec, bg, bm1, bm2;
+  std::vector<uint16_t *> Wb;
+  int64_t rows_cap = 0;
+  // activations: Xb (input, BF16), per hidden layer: Z (FP32 pre-act), Hb (BF16 act)
+  uint16_t *Xb = nullptr;
+  std::vector<float *> Z;
+  std::vector<uint16_t *> Hb;
+  float *Y = nullptr;       // output [rows x doutp]
+  float *T = nullptr;       // targets [rows x dout]
+  float *dY = nullptr, *dZ = nullptr, *dH = nullptr, *lrow = nullptr, *splitbuf = nullptr;
+  uint16_t *dYb = nullptr, *dZb = nullptr;
+  float *xin = nullptr;     // FP32 input staging [rows x din]
+  int64_t adam_t = 0;
+  int64_t splitbuf_len = 0;
+};
+
+namespace {
+
+bool mhip(nb_mlp *m, hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  char buf[512];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+  m->err = buf;
+  return false;
+}
+#define NB_MHIP(m, call)                               \
+  do {                                                 \
+    if (!mhip((m), (call), #call)) return NB_E_HIP;    \
+  } while (0)
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// C[M x N] = op(A) op(B) with optional split-K (partials summed in fixed order).
+int gemm(nb_mlp *m, bool ta, bool tb, const uint16_t *A, const uint16_t *B, float *C, int M,
+         int N, int K, int lda, int ldb, int epi, const float *bias, uint16_t *H, int splits) {
+  if (M % 64 || N % 64 || K % 16) {
+    m->err = "gemm: dimensions not padded";
+    return NB_E_INVALID;
+  }
+  if (splits < 1) splits = 1;
+  int kps = ((K / 16 + splits - 1) / splits) * 16;
+  splits = (K + kps - 1) / kps;
+  float *out = C;
+  if (splits > 1) {
+    if ((int64_t)splits * M * N > m->splitbuf_len) {
+      if (m->splitbuf) (void)hipFree(m->splitbuf);
+      m->splitbuf_len = (int64_t)splits * M * N;
+      NB_MHIP(m, hipMalloc((void **)&m->splitbuf, (size_t)m->splitbuf_len * sizeof(float)));
+    }
+    out = m->splitbuf;
+    epi = EPI_STORE;
+  }
+  dim3 grid((unsigned)(N / 64), (unsigned)(M / 64), (unsigned)splits);
+  if (!ta && !tb)
+    hipLaunchKernelGGL((k_gemm_bf16<false, false>), grid, dim3(256), 0, m->stream, A, B, out, M,
+                       N, K, kps, lda, ldb, epi, bias, H);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((k_gemm_bf16<true, false>), grid, dim3(256), 0, m->stream, A, B, out, M,
+                       N, K, kps, lda, ldb, epi, bias, H);
+  else if (!ta && tb)
+    hipLaunchKernelGGL((k_gemm_bf16<false, true>), grid, dim3(256), 0, m->stream, A, B, out, M,
+                       N, K, kps, lda, ldb, epi, bias, H);
+  else
+    return NB_E_INVALID;
+  NB_MHIP(m, hipGetLastError());
+  if (splits > 1) {
+    const int64_t cnt = (int64_t)M * N;
+    hipLaunchKernelGGL(k_splitk_sum, dim3(nblk(cnt)), dim3(256), 0, m->stream, m->splitbuf, C,
+                       cnt, splits);
+    NB_MHIP(m, hipGetLastError());
+  }
+  return NB_OK;
+}
+
+void free_rows(nb_mlp *m) {
+  void *p[] = {m->Xb, m->Y, m->T, m->dY, m->dZ, m->dH, m->lrow, m->dYb, m->dZb, m->xin};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  for (float *q : m->Z) (void)hipFree(q);
+  for (uint16_t *q : m->Hb) (void)hipFree(q);
+  m->Z.clear();
+  m->Hb.clear();
+  m->Xb = nullptr;
+  m->Y = m->T = m->dY = m->dZ = m->dH = m->lrow = m->xin = nullptr;
+  m->dYb = m->dZb = nullptr;
+  m->rows_cap = 0;
+}
+
+int ensure_rows(nb_mlp *m, int64_t rows) {
+  if (rows <= m->rows_cap) return NB_OK;
+  free_rows(m);
+  const int64_t R = round_up(std::max<int64_t>(rows, 64), 64);
+  const int hp = m->hid;
+  NB_MHIP(m, hipMalloc((void **)&m->Xb, (size_t)R * m->dinp * 2));
+  NB_MHIP(m, hipMalloc((void **)&m->xin, (size_t)R * m->din * 4));
+  for (int l = 0; l < m->layers; l++) {
+    float *z;
+    uint16_t *h;
+    NB_MHIP(m, hipMalloc((void **)&z, (size_t)R * hp * 4));
+    NB_MHIP(m, hipMalloc((void **)&h, (size_t)R * hp * 2));
+    m->Z.push_back(z);
+    m->Hb.push_back(h);
+  }
+  const int wmax = std::max(hp, m->doutp);
+  NB_MHIP(m, hipMalloc((void **)&m->Y, (size_t)R * m->doutp * 4));
+  NB_MHIP(m, hipMalloc((void **)&m->T, (size_t)R * m->dout * 4));
+  NB_MHIP(m, hipMalloc((void **)&m->dY, (size_t)R * m->doutp * 4));
+  NB_MHIP(m, hipMalloc((void **)&m->dYb, (size_t)R * wmax * 2));
+  NB_MHIP(m, hipMalloc((void **)&m->dZ, (size_t)R * wmax * 4));
+  NB_MHIP(m, hipMalloc((void **)&m->dZb, (size_t)R * wmax * 2));
+  NB_MHIP(m, hipMalloc((void **)&m->dH, (size_t)R * std::max(wmax, m->dinp) * 4));
+  NB_MHIP(m, hipMalloc((void **)&m->lrow, (size_t)R * 4));
+  m->rows_cap = R;
+  return NB_OK;
+}
+
+// Forward pass over `rows` (multiple of 64) already in Xb; output FP32 in Y [rows x doutp].
+int forward(nb_mlp *m, int64_t rows) {
+  const uint16_t *in = m->Xb;
+  int ld = m->dinp;
+  for (int l = 0; l < m->layers; l++) {
+    int rc = gemm(m, false, false, in, m->Wb[l], m->Z[l], (int)rows, m->hid, ld, ld, m->hid,
+                  EPI_BIAS_GELU, m->bvec[l], m->Hb[l], 1);
+    if (rc) return rc;
+    in = m->Hb[l];
+    ld = m->hid;
+  }
+  return gemm(m, false, false, in, m->Wb[m->layers], m->Y, (int)rows, m->doutp, ld, ld,
+              m->doutp, EPI_BIAS, m->bvec[m->layers], nullptr, 1);
+}
+
+// Backward pass of the MSE loss for `rows`; gradients into Wg / bg.  Targets in T.
+int backward(nb_mlp *m, int64_t rows) {
+  const int R = (int)rows;
+  hipLaunchKernelGGL(k_mse, dim3(nblk(R)), dim3(256), 0, m->stream, m->Y, m->doutp, m->T, R,
+                     m->dout, m->dY, m->dYb, m->lrow);
+  NB_MHIP(m, hipGetLastError());
+  const int splits = std::max(1, std::min(64, R / 256));
+  // output layer
+  const int L = m->layers;
+  const uint16_t *inL = L > 0 ? m->Hb[L - 1] : m->Xb;
+  const int ldL = L > 0 ? m->hid : m->dinp;
+  int rc = gemm(m, true, false, inL, m->dYb, m->Wg[L], ldL, m->doutp, R, ldL, m->doutp,
+                EPI_STORE, nullptr, nullptr, splits);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(m->doutp, 64)), dim3(64), 0, m->stream, m->dY, R,
+                     m->doutp, m->bg[L]);
+  NB_MHIP(m, hipGetLastError());
+  const uint16_t *gradb = m->dYb;  // BF16 gradient w.r.t. this layer's output
+  int gw = m->doutp;
+  for (int l = L - 1; l >= 0; l--) {
+    // dH_l = grad * W_{l+1}^T   [R x hid]
+    rc = gemm(m, false, true, gradb, m->Wb[l + 1], m->dH, R, m->hid, gw, gw, gw, EPI_STORE,
+              nullptr, nullptr, 1);
+    if (rc) return rc;
+    const int64_t cnt = (int64_t)R * m->hid;
+    hipLaunchKernelGGL(k_gelu_back, dim3(nblk(cnt)), dim3(256), 0, m->stream, m->dH, m->Z[l],
+                       m->dZ, m->dZb, cnt);
+    NB_MHIP(m, hipGetLastError());
+    const uint16_t *in = l > 0 ? m->Hb[l - 1] : m->Xb;
+    const int ld = l > 0 ? m->hid : m->dinp;
+    rc = gemm(m, true, false, in, m->dZb, m->Wg[l], ld, m->hid, R, ld, m->hid, EPI_STORE,
+              nullptr, nullptr, splits);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_colsum, dim3(nblk(m->hid, 64)), dim3(64), 0, m->stream, m->dZ, R,
+                       m->hid, m->bg[l]);
+    NB_MHIP(m, hipGetLastError());
+    gradb = m->dZb;
+    gw = m->hid;
+  }
+  return NB_OK;
+}
+
+int adam_step(nb_mlp *m, float lr) {
+  m->adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)m->adam_t), c2 = 1.f - powf(b2, (float)m->adam_t);
+  for (int l = 0; l <= m->layers; l++) {
+    const int64_t nw = (int64_t)m->in_dim[l] * m->out_dim[l];
+    hipLaunchKernelGGL(k_adam, dim3(nblk(nw)), dim3(256), 0, m->stream, m->W[l], m->Wg[l],
+                       m->Wm1[l], m->Wm2[l], m->Wb[l], nw, lr, b1, b2, 1e-8f, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(m->out_dim[l])), dim3(256), 0, m->stream, m->bvec[l],
+                       m->bg[l], m->bm1[l], m->bm2[l], (uint16_t *)nullptr,
+                       (int64_t)m->out_dim[l], lr, b1, b2, 1e-8f, c1, c2);
+    NB_MHIP(m, hipGetLastError());
+  }
+  return NB_OK;
+}
+
+// Upload FP32 host rows [rows x din] and convert/pad to Xb.
+int load_inputs(nb_mlp *m, const float *x, int64_t rows, int64_t rows_pad) {
+  NB_MHIP(m, hipMemsetAsync(m->xin, 0, (size_t)rows_pad * m->din * 4, m->stream));
+  NB_MHIP(m, hipMemcpyAsync(m->xin, x, (size_t)rows * m->din * 4, hipMemcpyHostToDevice,
+                            m->stream));
+  hipLaunchKernelGGL(k_pad_rows_bf16, dim3(nblk(rows_pad * m->dinp)), dim3(256), 0, m->stream,
+                     m->xin, (int)rows_pad, m->din, m->din, m->Xb, m->dinp);
+  NB_MHIP(m, hipGetLastError());
+  return NB_OK;
+}
+
+}  // namespace
+
+// Exposed to nb_sur.hip (same library): run the forward pass on device-resident FP32 features.
+namespace nb {
+int mlp_forward_device(nb_mlp *m, const float *x_dev, int64_t rows, const float **y_dev,
+                       int *ldy, hipStream_t st) {
+  hipStream_t keep = m->stream;
+  m->stream = st;
+  int rc = ensure_rows(m, rows);
+  const int64_t rp = round_up(std::max<int64_t>(rows, 64), 64);
+  if (!rc) {
+    hipLaunchKernelGGL(k_pad_rows_bf16, dim3(nblk(rp * m->dinp)), dim3(256), 0, st, x_dev,
+                       (int)rows, m->din, m->din, m->Xb, m->dinp);
+    rc = forward(m, rp);
+  }
+  m->stream = keep;
+  *y_dev = m->Y;
+  *ldy = m->doutp;
+  return rc;
+}
+
+// One Adam step on device-resident rows (features FP32 [rows x din], targets FP32 [rows x dout]).
+int mlp_train_step_device(nb_mlp *m, const float *x_dev, const float *t_dev, int64_t rows,
+                          float lr, hipStream_t st) {
+  hipStream_t keep = m->stream;
+  m->stream = st;
+  int rc = ensure_rows(m, rows);
+  if (!rc && rows % 64) rc = NB_E_INVALID;
+  if (!rc) {
+    hipLaunchKernelGGL(k_pad_rows_bf16, dim3(nblk(rows * m->dinp)), dim3(256), 0, st, x_dev,
+                       (int)rows, m->din, m->din, m->Xb, m->dinp);
+    if (hipMemcpyAsync(m->T, t_dev, (size_t)rows * m->dout * 4, hipMemcpyDeviceToDevice, st) !=
+        hipSuccess)
+      rc = NB_E_HIP;
+  }
+  if (!rc) rc = forward(m, rows);
+  if (!rc) rc = backward(m, rows);
+  if (!rc) rc = adam_step(m, lr);
+  m->stream = keep;
+  return rc;
+}
+int mlp_dims(const nb_mlp *m, int *din, int *dout) {
+  *din = m->din;
+  *dout = m->dout;
+  return NB_OK;
+}
+}  // namespace nb
+
+extern "C" {
+
+int nb_mlp_create(int32_t device, int32_t din, int32_t hidden, int32_t layers, int32_t dout,
+                  uint64_t seed, nb_mlp **out) {
+  if (!out) return NB_E_INVALID;
+  *out = nullptr;
+  if (din < 1 || din > 1024 || hidden < 64 || hidden % 64 || hidden > 1024 || layers < 1 ||
+      layers > 8 || dout < 1 || dout > 64)
+    return NB_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return NB_E_HIP;
+  nb_mlp *m = new (std::nothrow) nb_mlp();
+  if (!m) return NB_E_NOMEM;
+  m->device = device;
+  m->din = din;
+  m->dinp = (int)round_up(din, 64);  // the layer-0 weight gradient has M = dinp
+  m->hid = hidden;
+  m->layers = layers;
+  m->dout = dout;
+  m->doutp = 64;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete m;
+    return NB_E_HIP;
+  }
+  std::mt19937_64 rng(seed);
+  for (int l = 0; l <= layers; l++) {
+    const int in = l == 0 ? m->dinp : hidden;
+    const int o = l == layers ? m->doutp : hidden;
+    const int in_real = l == 0 ? din : hidden;
+    const int o_real = l == layers ? dout : hidden;
+    m->in_dim.push_back(in);
+    m->out_dim.push_back(o);
+    // Glorot-uniform on the real fan-in/out; padded rows/columns stay zero
+    std::vector<float> w((size_t)in * o, 0.f), z((size_t)std::max(in * o, o), 0.f);
+    const double lim = std::sqrt(6.0 / (in_real + o_real));
+    std::uniform_real_distribution<double> U(-lim, lim);
+    for (int i = 0; i < in_real; i++)
+      for (int j = 0; j < o_real; j++) w[(size_t)i * o + j] = (float)U(rng);
+    float *W, *Wg, *M1, *M2, *b, *bg, *bm1, *bm2;
+    uint16_t *Wb;
+    const size_t nw = (size_t)in * o;
+    bool ok = hipMalloc((void **)&W, nw * 4) == hipSuccess &&
+              hipMalloc((void **)&Wg, nw * 4) == hipSuccess &&
+              hipMalloc((void **)&M1, nw * 4) == hipSuccess &&
+              hipMalloc((void **)&M2, nw * 4) == hipSuccess &&
+              hipMalloc((void **)&Wb, nw * 2) == hipSuccess &&
+              hipMalloc((void **)&b, o * 4) == hipSuccess &&
+              hipMalloc((void **)&bg, o * 4) == hipSuccess &&
+              hipMalloc((void **)&bm1, o * 4) == hipSuccess &&
+              hipMalloc((void **)&bm2, o * 4) == hipSuccess;
+    if (!ok) {
+      nb_mlp_destroy(m);
+      return NB_E_NOMEM;
+    }
+    m->W.push_back(W);
+    m->Wg.push_back(Wg);
+    m->Wm1.push_back(M1);
+    m->Wm2.push_back(M2);
+    m->Wb.push_back(Wb);
+    m->bvec.push_back(b);
+    m->bg.push_back(bg);
+    m->bm1.push_back(bm1);
+    m->bm2.push_back(bm2);
+    (void)hipMemcpy(W, w.data(), nw * 4, hipMemcpyHostToDevice);
+    (void)hipMemset(M1, 0, nw * 4);
+    (void)hipMemset(M2, 0, nw * 4);
+    (void)hipMemset(Wg, 0, nw * 4);
+    (void)hipMemset(b, 0, o * 4);
+    (void)hipMemset(bg, 0, o * 4);
+    (void)hipMemset(bm1, 0, o * 4);
+    (void)hipMemset(bm2, 0, o * 4);
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk((int64_t)nw)), dim3(256), 0, m->stream, W, Wb,
+                       (int64_t)nw);
+  }
+  if (hipStreamSynchronize(m->stream) != hipSuccess) {
+    nb_mlp_destroy(m);
+    return NB_E_HIP;
+  }
+  *out = m;
+  return NB_OK;
+}
+
+void nb_mlp_destroy(nb_mlp *m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  free_rows(m);
+  for (auto *v : {&m->W, &m->Wg, &m->Wm1, &m->Wm2, &m->bvec, &m->bg, &m->bm1, &m->bm2})
+    for (float *q : *v) (void)hipFree(q);
+  for (uint16_t *q : m->Wb) (void)hipFree(q);
+  if (m->splitbuf) (void)hipFree(m->splitbuf);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+const char *nb_mlp_last_error(const nb_mlp *m) { return m ? m->err.c_str() : "NULL mlp"; }
+
+// Parameters in the unpadded layout: for each layer W [in_real x out_real] row-major, then b.
+int nb_mlp_param_count(const nb_mlp *m, int64_t *count) {
+  if (!m || !count) return NB_E_INVALID;
+  int64_t c = 0;
+  for (int l = 0; l <= m->layers; l++) {
+    const int in = l == 0 ? m->din : m->hid, o = l == m->layers ? m->dout : m->hid;
+    c += (int64_t)in * o + o;
+  }
+  *count = c;
+  return NB_OK;
+}
+
+static int xfer_params(nb_mlp *m, float *host, int64_t count, bool to_dev, bool grads) {
+  int64_t need = 0;
+  nb_mlp_param_count(m, &need);
+  if (count != need) {
+    m->err = "parameter count mismatch";
+    return NB_E_INVALID;
+  }
+  NB_MHIP(m, hipSetDevice(m->device));
+  NB_MHIP(m, hipStreamSynchronize(m->stream));
+  int64_t off = 0;
+  for (int l = 0; l <= m->layers; l++) {
+    const int in = l == 0 ? m->din : m->hid, o = l == m->layers ? m->dout : m->hid;
+    const int op = m->out_dim[l];
+    float *W = grads ? m->Wg[l] : m->W[l];
+    float *b = grads ? m->bg[l] : m->bvec[l];
+    std::vector<float> full((size_t)m->in_dim[l] * op, 0.f), bf((size_t)op, 0.f);
+    if (to_dev) {
+      for (int i = 0; i < in; i++)
+        for (int j = 0; j < o; j++) full[(size_t)i * op + j] = host[off + (int64_t)i * o + j];
+      for (int j = 0; j < o; j++) bf[j] = host[off + (int64_t)in * o + j];
+      NB_MHIP(m, hipMemcpy(W, full.data(), full.size() * 4, hipMemcpyHostToDevice));
+      NB_MHIP(m, hipMemcpy(b, bf.data(), bf.size() * 4, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk((int64_t)full.size())), dim3(256), 0,
+                         m->stream, W, m->Wb[l], (int64_t)full.size());
+      NB_MHIP(m, hipStreamSynchronize(m->stream));
+    } else {
+      NB_MHIP(m, hipMemcpy(full.data(), W, full.size() * 4, hipMemcpyDeviceToHost));
+      NB_MHIP(m, hipMemcpy(bf.data(), b, bf.size() * 4, hipMemcpyDeviceToHost));
+      for (int i = 0; i < in; i++)
+        for (int j = 0; j < o; j++) host[off + (int64_t)i * o + j] = full[(size_t)i * op + j];
+      for (int j = 0; j < o; j++) host[off + (int64_t)in * o + j] = bf[j];
+    }
+    off += (int64_t)in * o + o;
+  }
+  return NB_OK;
+}
+
+int nb_mlp_set_params(nb_mlp *m, const float *p, int64_t count) {
+  if (!m || !p) return NB_E_INVALID;
+  int rc = xfer_params(m, const_cast<float *>(p), count, true, false);
+  m->adam_t = 0;
+  return rc;
+}
+
+int nb_mlp_get_params(nb_mlp *m, float *p, int64_t count) {
+  if (!m || !p) return NB_E_INVALID;
+  return xfer_params(m, p, count, false, false);
+}
+
+int nb_mlp_forward(nb_mlp *m, const float *x, int64_t rows, float *y) {
+  if (!m || (rows > 0 && (!x || !y)) || rows < 0) return NB_E_INVALID;
+  if (rows == 0) return NB_OK;
+  NB_MHIP(m, hipSetDevice(m->device));
+  const int64_t rp = round_up(rows, 64);
+  int rc = ensure_rows(m, rp);
+  if (!rc) rc = load_inputs(m, x, rows, rp);
+  if (!rc) rc = forward(m, rp);
+  if (rc) return rc;
+  std::vector<float> Y((size_t)rp * m->doutp);
+  NB_MHIP(m, hipMemcpyAsync(Y.data(), m->Y, Y.size() * 4, hipMemcpyDeviceToHost, m->stream));
+  NB_MHIP(m, hipStreamSynchronize(m->stream));
+  for (int64_t r = 0; r < rows; r++)
+    for (int c = 0; c < m->dout; c++) y[r * m->dout + c] = Y[(size_t)r * m->doutp + c];
+  return NB_OK;
+}
+
+// Gradient of the MSE loss over `rows` (a multiple of 64) w.r.t. the parameters (unpadded layout).
+int nb_mlp_grad(nb_mlp *m, const float *x, const float *t, int64_t rows, float *grad,
+                float *loss) {
+  if (!m || !x || !t || !grad || rows <= 0 || rows % 64) return NB_E_INVALID;
+  NB_MHIP(m, hipSetDevice(m->device));
+  int rc = ensure_rows(m, rows);
+  if (!rc) rc = load_inputs(m, x, rows, rows);
+  if (rc) return rc;
+  NB_MHIP(m, hipMemcpyAsync(m->T, t, (size_t)rows * m->dout * 4, hipMemcpyHostToDevice,
+                            m->stream));
+  rc = forward(m, rows);
+  if (!rc) rc = backward(m, rows);
+  if (rc) return rc;
+  if (loss) {
+    std::vector<float> l((size_t)rows);
+    NB_MHIP(m, hipMemcpyAsync(l.data(), m->lrow, l.size() * 4, hipMemcpyDeviceToHost,
+                              m->stream));
+    NB_MHIP(m, hipStreamSynchronize(m->stream));
+    double s = 0;
+    for (float v : l) s += v;
+    *loss = (float)(s / ((double)rows * m->dout));
+  }
+  int64_t cnt = 0;
+  nb_mlp_param_count(m, &cnt);
+  return xfer_params(m, grad, cnt, false, true);
+}
+
+// Adam training on host-provided rows: `steps` minibatches of `batch` rows (multiple of 64) drawn
+// in order (wrapping) from the data.  Returns the mean loss of the last 10% of steps.
+int nb_mlp_train(nb_mlp *m, const float *x, const float *t, int64_t rows, int32_t steps,
+                 int32_t batch, float lr, float *loss_out) {
+  if (!m || !x || !t || rows < batch || batch <= 0 || batch % 64 || steps < 0)
+    return NB_E_INVALID;
+  NB_MHIP(m, hipSetDevice(m->device));
+  float *dx = nullptr, *dt = nullptr;
+  NB_MHIP(m, hipMalloc((void **)&dx, (size_t)rows * m->din * 4));
+  if (hipMalloc((void **)&dt, (size_t)rows * m->dout * 4) != hipSuccess) {
+    (void)hipFree(dx);
+    return NB_E_NOMEM;
+  }
+  int rc = NB_OK;
+  if (hipMemcpy(dx, x, (size_t)rows * m->din * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dt, t, (size_t)rows * m->dout * 4, hipMemcpyHostToDevice) != hipSuccess)
+    rc = NB_E_HIP;
+  double lsum = 0;
+  int lcount = 0;
+  const int64_t nb = rows / batch;
+  std::vector<float> l((size_t)batch);
+  for (int s = 0; s < steps && !rc; s++) {
+    const int64_t b0 = (s % nb) * batch;
+    rc = nb::mlp_train_step_device(m, dx + b0 * m->din, dt + b0 * m->dout, batch, lr, m->stream);
+    if (!rc && s >= steps - std::max(1, steps / 10)) {
+      if (hipMemcpyAsync(l.data(), m->lrow, l.size() * 4, hipMemcpyDeviceToHost, m->stream) !=
+              hipSuccess ||
+          hipStreamSynchronize(m->stream) != hipSuccess) {
+        rc = NB_E_HIP;
+        break;
+      }
+      double q = 0;
+      for (float v : l) q += v;
+      lsum += q / ((double)batch * m->dout);
+      lcount++;
+    }
+  }
+  (void)hipStreamSynchronize(m->stream);
+  (void)hipFree(dx);
+  (void)hipFree(dt);
+  if (loss_out) *loss_out = lcount ? (float)(lsum / lcount) : 0.f;
+  return rc;
+}
+
+}  // extern "C"

@@ -186,3 +186,87 @@ class NBodyGPU:
         cfg = (C.c_int32 * 5)()
         self._check(lib.nb_autotune(self._ctx, int(reps), cfg))
         return dict(zip(("ipl", "tile_j", "jsplit", "block", "variant"), list(cfg)))
+
+
+class NBodyBatch:
+    """Many independent simulations in one persistent, graph-captured launch (nb_batch_*)."""
+
+    def __init__(self, G=1.0, eps=0.01, device=0):
+        p = L.NbParams()
+        check(lib.nb_params_init(C.byref(p)))
+        p.G, p.eps, p.device = float(G), float(eps), int(device)
+        p.overlap_cutoff = 0
+        p.collisions = 0
+        self._b = C.c_void_p()
+        rc = lib.nb_batch_create(C.byref(p), C.byref(self._b))
+        if rc != L.NB_OK:
+            raise L.NBodyError(rc, "nb_batch_create failed")
+        self.offsets = None
+
+    def _check(self, rc):
+        if rc != L.NB_OK:
+            raise L.NBodyError(rc, lib.nb_batch_last_error(self._b).decode())
+
+    def close(self):
+        if self._b:
+            lib.nb_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, sims):
+        """sims: list of Bodies."""
+        from .gen import concat
+        b = concat(*sims)
+        off = np.zeros(len(sims) + 1, np.int64)
+        off[1:] = np.cumsum([s.n for s in sims])
+        self.offsets = off
+        self.n = b.n
+        self._check(lib.nb_batch_load(self._b, len(sims), off.ctypes.data_as(L._i64p), _dp(b.x),
+                                      _dp(b.y), _dp(b.z), _dp(b.vx), _dp(b.vy), _dp(b.vz),
+                                      _dp(b.m), b.flags.ctypes.data_as(L._u8p)))
+        self._all = b
+        return self
+
+    def step(self, dt, R=1.0, nsteps=1):
+        st = L.NbStats()
+        self._check(lib.nb_batch_step(self._b, float(dt), float(R), int(nsteps), C.byref(st)))
+        return st.as_dict()
+
+    def set_graph(self, on):
+        self._check(lib.nb_batch_set_graph(self._b, 1 if on else 0))
+
+    def forces(self):
+        f = np.empty((3, self.n), np.float64)
+        self._check(lib.nb_batch_forces(self._b, _dp(f[0]), _dp(f[1]), _dp(f[2])))
+        return f.T.copy()
+
+    def store(self):
+        b = self._all.copy()
+        self._check(lib.nb_batch_store(self._b, _dp(b.x), _dp(b.y), _dp(b.z), _dp(b.vx),
+                                       _dp(b.vy), _dp(b.vz), b.flags.ctypes.data_as(L._u8p)))
+        return b
+
+    def shapes(self):
+        s = (C.c_int32 * 3)()
+        self._check(lib.nb_batch_get_shapes(self._b, s))
+        return list(s)
+
+    def set_shapes(self, ipl3):
+        s = (C.c_int32 * 3)(*ipl3)
+        self._check(lib.nb_batch_set_shapes(self._b, s))
+
+    def autotune(self, reps=2):
+        s = (C.c_int32 * 3)()
+        self._check(lib.nb_batch_autotune(self._b, int(reps), s))
+        return list(s)
