@@ -119,6 +119,17 @@ SIGNATURES = [
     ("nb_batch_get_shapes", C.c_int, [_ctx, _i32p]),
     ("nb_batch_set_shapes", C.c_int, [_ctx, _i32p]),
     ("nb_batch_autotune", C.c_int, [_ctx, C.c_int32, _i32p]),
+    ("nb_mlp_create", C.c_int,
+     [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_ctx)]),
+    ("nb_mlp_destroy", None, [_ctx]),
+    ("nb_mlp_last_error", C.c_char_p, [_ctx]),
+    ("nb_mlp_param_count", C.c_int, [_ctx, _i64p]),
+    ("nb_mlp_set_params", C.c_int, [_ctx, _fp, C.c_int64]),
+    ("nb_mlp_get_params", C.c_int, [_ctx, _fp, C.c_int64]),
+    ("nb_mlp_forward", C.c_int, [_ctx, _fp, C.c_int64, _fp]),
+    ("nb_mlp_grad", C.c_int, [_ctx, _fp, _fp, C.c_int64, _fp, _fp]),
+    ("nb_mlp_train", C.c_int,
+     [_ctx, _fp, _fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, _fp]),
 ]
 
 
