@@ -228,6 +228,55 @@ int nb_mlp_grad(nb_mlp *m, const float *x, const float *t, int64_t rows, float *
 int nb_mlp_train(nb_mlp *m, const float *x, const float *t, int64_t rows, int32_t steps,
                  int32_t batch, float lr, float *loss_out);
 
+/* ---- learned far-field surrogate for the softened force ------------------------------------ */
+/* No counterpart in the reference: its sub-quadratic design, the Barnes-Hut octree
+ * (cmd/body/octree.go), is never built.  When enabled on a context (FP32 path, eps > 0, no
+ * collisions or overlap cutoff), nb_step and nb_compute_forces replace the O(N^2) force with an
+ * exact near field over adjacent cells of a stretched 32^3 grid, per-body quadrupoles of the
+ * cells at Chebyshev distance 2..shell, a second-order Taylor far expansion, and an MLP residual;
+ * every check_every steps a sampled direct-sum error above tol makes the step fall back to the
+ * direct sum.  nb_stats.interactions still counts the N(N-1) pairs the step stands for. */
+typedef struct nb_sur_params {
+  int32_t grid;          /* cells per axis, 4..32 (32) */
+  int32_t shell;         /* per-body multipole shell depth K, 1..6 (2) */
+  double r0;             /* stretch radius; 0 = 1.5 x the half-mass radius at enable time */
+  int32_t use_mlp;       /* apply the learned residual once trained (1) */
+  int32_t check_every;   /* steps between sampled error checks; 0 = never (1) */
+  int32_t check_samples; /* bodies per check (256) */
+  int32_t pad0;
+  double tol;            /* sampled relative error above which the step is redone (1e-3) */
+  uint64_t seed;
+  int32_t reserved[8];
+} nb_sur_params;
+
+typedef struct nb_sur_stats {
+  int64_t steps;           /* surrogate force phases run */
+  int64_t checks;          /* sampled error checks */
+  int64_t fallbacks;       /* checks that redid the step with the direct sum */
+  double last_error;       /* relative error of the last check */
+  double near_pairs_total; /* exact near-field pairs evaluated so far */
+  int64_t nonempty_cells;  /* at the last step */
+  int64_t train_rows;      /* collected training rows */
+  double train_loss;       /* last training loss (normalised units) */
+  double r0;               /* stretch radius in use */
+  int32_t trained;
+  int32_t pad1;
+} nb_sur_stats;
+
+int nb_sur_params_init(nb_sur_params *p);
+int nb_sur_enable(nb_ctx *ctx, const nb_sur_params *p);
+int nb_sur_disable(nb_ctx *ctx);
+/* Append `samples` training rows from the loaded state (labels from the FP32 direct sum). */
+int nb_sur_collect(nb_ctx *ctx, int64_t samples, uint64_t seed);
+/* Fit the MLP on the collected rows with the device Adam (steps minibatches of batch rows). */
+int nb_sur_fit(nb_ctx *ctx, int32_t steps, int32_t batch, float lr, float *loss);
+/* Sampled relative force error of the surrogate on the loaded state, with (1) or without (0) the
+ * learned residual. */
+int nb_sur_error(nb_ctx *ctx, int32_t samples, int32_t use_mlp, double *err);
+int nb_sur_get_stats(nb_ctx *ctx, nb_sur_stats *out);
+int nb_sur_save(nb_ctx *ctx, const char *path);
+int nb_sur_load(nb_ctx *ctx, const char *path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,40 @@ class NbDiag(C.Structure):
     ]
 
 
+class NbSurParams(C.Structure):
+    _fields_ = [
+        ("grid", C.c_int32),
+        ("shell", C.c_int32),
+        ("r0", C.c_double),
+        ("use_mlp", C.c_int32),
+        ("check_every", C.c_int32),
+        ("check_samples", C.c_int32),
+        ("pad0", C.c_int32),
+        ("tol", C.c_double),
+        ("seed", C.c_uint64),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class NbSurStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("checks", C.c_int64),
+        ("fallbacks", C.c_int64),
+        ("last_error", C.c_double),
+        ("near_pairs_total", C.c_double),
+        ("nonempty_cells", C.c_int64),
+        ("train_rows", C.c_int64),
+        ("train_loss", C.c_double),
+        ("r0", C.c_double),
+        ("trained", C.c_int32),
+        ("pad1", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad1"}
+
+
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _i32p = C.POINTER(C.c_int32)
@@ -130,6 +164,15 @@ SIGNATURES = [
     ("nb_mlp_grad", C.c_int, [_ctx, _fp, _fp, C.c_int64, _fp, _fp]),
     ("nb_mlp_train", C.c_int,
      [_ctx, _fp, _fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, _fp]),
+    ("nb_sur_params_init", C.c_int, [C.POINTER(NbSurParams)]),
+    ("nb_sur_enable", C.c_int, [_ctx, C.POINTER(NbSurParams)]),
+    ("nb_sur_disable", C.c_int, [_ctx]),
+    ("nb_sur_collect", C.c_int, [_ctx, C.c_int64, C.c_uint64]),
+    ("nb_sur_fit", C.c_int, [_ctx, C.c_int32, C.c_int32, C.c_float, _fp]),
+    ("nb_sur_error", C.c_int, [_ctx, C.c_int32, C.c_int32, _dp]),
+    ("nb_sur_get_stats", C.c_int, [_ctx, C.POINTER(NbSurStats)]),
+    ("nb_sur_save", C.c_int, [_ctx, C.c_char_p]),
+    ("nb_sur_load", C.c_int, [_ctx, C.c_char_p]),
 ]
 
 

@@ -29,6 +29,8 @@ static int fail(Ctx *c, int code, const std::string &msg) {
   return code;
 }
 
+int fail_msg(Ctx *c, int code, const char *msg) { return fail(c, code, msg); }
+
 template <class T>
 static hipError_t dalloc(T **p, size_t count) {
   if (*p) {
@@ -115,7 +117,7 @@ static int prepare_shape(Ctx *c) {
   return NB_OK;
 }
 
-static ForceArgs force_args(Ctx *c) {
+ForceArgs make_force_args(Ctx *c) {
   ForceArgs a;
   a.js = c->js;
   a.rmax_tile = c->rmax_tile;
@@ -135,11 +137,13 @@ static ForceArgs force_args(Ctx *c) {
   a.recs = c->recs;
   a.rec_cap = (int64_t)c->acc_split * c->cap;
   a.ctr = c->ctr;
+  a.run_flag = nullptr;
   return a;
 }
 
 // Force phase of one cycle (force + detection + sorted pair list).
 static int force_phase(Ctx *c, bool timed, int ev_slot) {
+  if (c->sur_on) return sur_force_phase(c, timed, ev_slot);
   NB_HIP(c, launch_zero_counters(c->ctr, 0, c->stream));
   if (c->p.precision == NB_PREC_FP64) {
     Force64Args a;
@@ -160,7 +164,7 @@ static int force_phase(Ctx *c, bool timed, int ev_slot) {
     NB_HIP(c, launch_force_f64(a, c->stream));
     if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], c->stream));
   } else {
-    ForceArgs a = force_args(c);
+    ForceArgs a = make_force_args(c);
     if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot], c->stream));
     NB_HIP(c, launch_force_f32(c->shape, a, c->stream));
     if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], c->stream));
@@ -194,7 +198,9 @@ static int force_phase(Ctx *c, bool timed, int ev_slot) {
   return NB_OK;
 }
 
-static int split_now(const Ctx *c) { return c->p.precision == NB_PREC_FP64 ? 1 : c->shape.jsplit; }
+static int split_now(const Ctx *c) {
+  return (c->p.precision == NB_PREC_FP64 || c->sur_on) ? 1 : c->shape.jsplit;
+}
 
 static int64_t count_active(int64_t n, const uint8_t *flags) {
   int64_t a = 0;
@@ -207,7 +213,6 @@ static int64_t count_active(int64_t n, const uint8_t *flags) {
 
 using namespace nb;
 
-struct nb_ctx : public nb::Ctx {};
 
 extern "C" {
 
@@ -276,6 +281,8 @@ void nb_destroy(nb_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  sur_free(c->sur);
+  c->sur = nullptr;
   free_all(c);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -669,7 +676,7 @@ int nb_autotune(nb_ctx *c, int32_t reps, int32_t *cfg) {
     if (dup) continue;
     c->shape = s;
     if (prepare_shape(c)) continue;
-    ForceArgs a = force_args(c);
+    ForceArgs a = make_force_args(c);
     if (launch_force_f32(s, a, c->stream) != hipSuccess) continue;  // warm-up
     NB_HIP(c, hipEventRecord(e0, c->stream));
     for (int k = 0; k < reps; k++) (void)launch_force_f32(s, a, c->stream);

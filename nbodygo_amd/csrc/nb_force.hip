@@ -187,6 +187,7 @@ __device__ __forceinline__ void store_targets(const TargetState<IPL> &T, const F
 // VALU ops take the source operands from SGPRs; no LDS and no barriers.
 template <int IPL, bool DETECT>
 __global__ __launch_bounds__(256) void k_force_scalar(ForceArgs a) {
+  if (a.run_flag && *a.run_flag == 0u) return;  // conditional launch (surrogate fallback)
   TargetState<IPL> T;
   const int stride = blockDim.x;
   const int base = blockIdx.x * blockDim.x * IPL + threadIdx.x;
@@ -219,6 +220,7 @@ constexpr int kLdsTileMax = 1024;
 template <int IPL, bool DETECT>
 __global__ __launch_bounds__(256) void k_force_lds(ForceArgs a) {
   __shared__ float4 sh[kLdsTileMax];
+  if (a.run_flag && *a.run_flag == 0u) return;  // conditional launch (surrogate fallback)
   TargetState<IPL> T;
   const int stride = blockDim.x;
   const int base = blockIdx.x * blockDim.x * IPL + threadIdx.x;

@@ -31,7 +31,10 @@ struct DevCounters {
   uint64_t fixup_tiles;
   uint32_t pair_overflow;
   uint32_t pad;
+  uint64_t sur_fallbacks;    // surrogate steps redone by direct summation
 };
+
+struct Sur;  // far-field surrogate state (nb_sur.hip)
 
 // FP32 force kernel launch shape.
 struct ForceShape {
@@ -85,6 +88,9 @@ struct Ctx {
   bool shape_tuned = false;
   bool timing = false;
   std::vector<hipEvent_t> ev;
+
+  Sur *sur = nullptr;
+  bool sur_on = false;
 };
 
 bool hip_ok(Ctx *c, hipError_t e, const char *what);
@@ -123,8 +129,13 @@ struct ForceArgs {
   int2 *recs;           // (slice, target) of every target with flagged tiles
   int64_t rec_cap;      // entries in recs (slices * padded bodies)
   DevCounters *ctr;
+  const uint32_t *run_flag;  // if set, the launch does nothing unless *run_flag != 0
 };
 hipError_t launch_force_f32(const ForceShape &s, const ForceArgs &a, hipStream_t st);
+ForceArgs make_force_args(Ctx *c);
+int fail_msg(Ctx *c, int code, const char *msg);
+void sur_free(Sur *s);
+int sur_force_phase(Ctx *c, bool timed, int ev_slot);
 
 struct FixupArgs {
   const double *x, *y, *z, *m, *r;
@@ -215,3 +226,6 @@ hipError_t launch_render(const double *x, const double *y, const double *z, cons
                          int n, float *xyz, hipStream_t st);
 
 }  // namespace nb
+
+// The opaque handle of the C-ABI is the context itself.
+struct nb_ctx : public nb::Ctx {};

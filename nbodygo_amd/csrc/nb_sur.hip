@@ -1,0 +1,1274 @@
+// nb_sur.hip -- learned far-field surrogate for the softened force (no counterpart in the
+// reference, whose own far-field design, the Barnes-Hut octree of cmd/body/octree.go, is never
+// built).  It replaces the O(N^2) force phase with
+//
+//   exact near field   FP32 pair sums over the 27 cells adjacent to a body's cell
+//   multipole shell    per-body monopole + quadrupole of every cell at Chebyshev distance 2..K
+//   far expansion      per-cell field, Jacobian and Hessian at the cell's centre of mass from all
+//                      cells beyond distance K (monopole + quadrupole field, monopole derivatives),
+//                      applied to each body by a second-order Taylor step
+//   learned residual   an MLP (BF16 MFMA, nb_mlp.hip) maps scale-free features of the body's cell
+//                      and position to the remaining far-field error, trained on direct-sum data
+//   fallback           every check_every steps the force of check_samples bodies is recomputed by
+//                      direct summation; if the sampled relative error exceeds tol, the step's
+//                      force is redone with the plain kernel (a device flag, no host round trip)
+//
+// Grid: 32^3 cells in stretched coordinates u = d / (|d| + r0), d = x - centre of mass, which map
+// all of space into the unit ball, so no body falls outside the grid; cells are about r0/16 wide
+// at the centre and grow outward where the density falls.  Bodies are sorted by cell (stable
+// radix sort), so the three cells of a grid row are one contiguous run of sources.
+//
+// All reductions run in a fixed order; the result is deterministic.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include <rocprim/rocprim.hpp>
+
+#include "nb_internal.hpp"
+
+namespace nb {
+
+int mlp_forward_device(nb_mlp *m, const float *x_dev, int64_t rows, const float **y_dev,
+                       int *ldy, hipStream_t st);
+int mlp_train_step_device(nb_mlp *m, const float *x_dev, const float *t_dev, int64_t rows,
+                          float lr, hipStream_t st);
+
+constexpr int kCellRec = 16;  // COM(3), M, Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, radius, ix, iy, iz, n, 0
+constexpr int kFarRec = 20;   // a(3), J(6: xx yy zz xy xz yz), H(10), 0
+constexpr int kFeat = 30;     // MLP input width
+constexpr int kMaxGrid = 32;
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) f4v *cf4p;  // scalar-cache (uniform) loads
+typedef const __attribute__((address_space(4))) float *cfp;
+typedef const __attribute__((address_space(4))) int32_t *cip;
+
+struct Sur {
+  nb_sur_params p{};
+  int G = 32, ncell = 0, K = 2;
+  double r0 = 0;
+  int64_t cap = 0;
+  uint32_t *keys = nullptr, *keys2 = nullptr, *perm_in = nullptr, *perm = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+  int32_t *cstart = nullptr, *ccount = nullptr, *nelist = nullptr, *necount = nullptr;
+  float4 *sp = nullptr;
+  float *crec = nullptr, *crec_ne = nullptr, *fexp = nullptr, *part = nullptr, *feat = nullptr;
+  int2 *items = nullptr;
+  int32_t *nitems = nullptr;
+  int64_t items_cap = 0;
+  float4 *base = nullptr;
+  double *center = nullptr, *red = nullptr;
+  float *mu = nullptr, *sd = nullptr;
+  nb_mlp *mlp = nullptr;
+  bool trained = false;
+  float out_scale = 1.f;
+  float *tx = nullptr, *ty = nullptr;
+  int64_t trows = 0, tcap = 0;
+  int32_t *sidx = nullptr;
+  int S = 0;
+  double *sexact = nullptr, *serr = nullptr;
+  uint32_t *flag = nullptr;
+  int64_t steps = 0, fallbacks = 0, checks = 0;
+  double last_err = 0;
+  float train_loss = 0;
+  float *mu0 = nullptr, *sd1 = nullptr;  // identity normalisation (raw features)
+  double *near_pairs = nullptr;          // running count of exact near-field pairs
+};
+
+__device__ __forceinline__ bool act(uint8_t f) {
+  return (f & NB_F_EXISTS) && !(f & NB_F_FRAGMENTING);
+}
+
+// ---- centre of mass (two-stage, fixed order) ------------------------------------------------
+__global__ __launch_bounds__(256) void k_sur_com_part(const double *x, const double *y,
+                                                      const double *z, const double *m,
+                                                      const uint8_t *flags, int n, double *part) {
+  __shared__ double sh[4][256];
+  double s[4] = {0, 0, 0, 0};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    if (act(flags[i])) {
+      s[0] += m[i];
+      s[1] += m[i] * x[i];
+      s[2] += m[i] * y[i];
+      s[3] += m[i] * z[i];
+    }
+  for (int c = 0; c < 4; c++) sh[c][threadIdx.x] = s[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 4; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; c++) part[blockIdx.x * 4 + c] = sh[c][0];
+}
+
+__global__ __launch_bounds__(256) void k_sur_com_final(const double *part, int nb,
+                                                       double *center) {
+  __shared__ double sh[4][256];
+  double s[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += 256)
+    for (int c = 0; c < 4; c++) s[c] += part[b * 4 + c];
+  for (int c = 0; c < 4; c++) sh[c][threadIdx.x] = s[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 4; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double M = sh[0][0] > 0 ? sh[0][0] : 1.0;
+    center[0] = sh[1][0] / M;
+    center[1] = sh[2][0] / M;
+    center[2] = sh[3][0] / M;
+    center[3] = sh[0][0];
+  }
+}
+
+// ---- binning ---------------------------------------------------------------------------------
+__device__ __forceinline__ int cell_of(double dx, double dy, double dz, double r0, int G) {
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  const double s = 1.0 / (r + r0);
+  const double h = 0.5 * G;
+  int ix = (int)floor((dx * s + 1.0) * h), iy = (int)floor((dy * s + 1.0) * h),
+      iz = (int)floor((dz * s + 1.0) * h);
+  ix = min(max(ix, 0), G - 1);
+  iy = min(max(iy, 0), G - 1);
+  iz = min(max(iz, 0), G - 1);
+  return ix + G * (iy + G * iz);
+}
+
+__global__ void k_sur_keys(const double *x, const double *y, const double *z,
+                           const uint8_t *flags, int n, const double *center, double r0, int G,
+                           uint32_t *keys, uint32_t *perm_in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t key = (uint32_t)(G * G * G);
+  if (act(flags[i]))
+    key = (uint32_t)cell_of(x[i] - center[0], y[i] - center[1], z[i] - center[2], r0, G);
+  keys[i] = key;
+  perm_in[i] = (uint32_t)i;
+}
+
+// cstart[c] = first sorted position whose key is >= c, for c = 0..ncell (inactive bodies carry
+// key ncell and sort last, so cstart[ncell] = active bodies).  Binary search, no atomics.
+__global__ void k_sur_bounds(const uint32_t *keys2, int n, int ncell, int32_t *cstart) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncell) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int)keys2[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  cstart[c] = lo;
+}
+
+// Per-cell counts, the list of nonempty cells and the near-field work items (cell, 64-body chunk),
+// by one workgroup of 1024 threads walking the cells in rounds of 1024 (one cell per thread):
+// wave prefix sums, then the 16 wave totals, then a running carry -- a fixed-order scan.
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(1024) void k_sur_scan(const int32_t *cstart, int ncell,
+                                                   int32_t *ccount, int32_t *nelist,
+                                                   int32_t *necount, int2 *items,
+                                                   int32_t *nitems) {
+  __shared__ int32_t wa[16], wb[16], carry[2];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) carry[0] = carry[1] = 0;
+  __syncthreads();
+  for (int base = 0; base < ncell; base += 1024) {
+    const int c = base + t;
+    const int cnt = c < ncell ? cstart[c + 1] - cstart[c] : 0;
+    const int a = cnt > 0, b = (cnt + 63) >> 6;
+    const int ia = wave_incl_scan(a, lane), ib = wave_incl_scan(b, lane);
+    if (lane == 63) {
+      wa[w] = ia;
+      wb[w] = ib;
+    }
+    __syncthreads();
+    if (w == 0) {
+      const int va = wave_incl_scan(lane < 16 ? wa[lane] : 0, lane);
+      const int vb = wave_incl_scan(lane < 16 ? wb[lane] : 0, lane);
+      if (lane < 16) {
+        wa[lane] = va;
+        wb[lane] = vb;
+      }
+    }
+    __syncthreads();
+    const int offa = carry[0] + (w > 0 ? wa[w - 1] : 0) + ia - a;
+    const int offb = carry[1] + (w > 0 ? wb[w - 1] : 0) + ib - b;
+    if (c < ncell) {
+      ccount[c] = cnt;
+      if (a) nelist[offa] = c;
+      for (int j = 0; j < b; j++) items[offb + j] = make_int2(c, j);
+    }
+    __syncthreads();
+    if (t == 0) {
+      carry[0] += wa[15];
+      carry[1] += wb[15];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    *necount = carry[0];
+    *nitems = carry[1];
+  }
+}
+
+__global__ void k_sur_gather(const uint32_t *perm, const uint32_t *keys2, int n, int ncell,
+                             const double *x, const double *y, const double *z, const double *m,
+                             const double *center, float4 *sp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || (int)keys2[k] >= ncell) return;
+  const uint32_t i = perm[k];
+  sp[k] = make_float4((float)(x[i] - center[0]), (float)(y[i] - center[1]),
+                      (float)(z[i] - center[2]), (float)m[i]);
+}
+
+// Butterfly sum over the 64 lanes; every lane ends with the same bits (each step adds the two
+// operands in either order, and FP addition commutes).
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ double wmax(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// Cell moments about the cell's centre of mass (FP64 accumulation), one wave per nonempty cell:
+// M, COM, traceless quadrupole Q_ab = sum m (3 d_a d_b - |d|^2 delta_ab), radius = max |d|.
+// Written by cell index (crec, for the per-body shell) and by nonempty rank (crec_ne, the
+// contiguous source stream of the far expansion).
+__global__ __launch_bounds__(64) void k_sur_moments(const float4 *sp, const int32_t *cstart,
+                                                    const int32_t *nelist,
+                                                    const int32_t *necount, int G, float *crec,
+                                                    float *crec_ne) {
+  const int t = blockIdx.x;
+  if (t >= *necount) return;
+  const int c = nelist[t];
+  const int s = cstart[c], e = cstart[c + 1], lane = threadIdx.x;
+  double M = 0, X = 0, Y = 0, Z = 0, gx = 0, gy = 0, gz = 0;
+  for (int k = s + lane; k < e; k += 64) {
+    const float4 b = sp[k];
+    M += b.w;
+    X += (double)b.w * b.x;
+    Y += (double)b.w * b.y;
+    Z += (double)b.w * b.z;
+    gx += b.x;
+    gy += b.y;
+    gz += b.z;
+  }
+  M = wsum(M);
+  if (M > 0) {
+    X = wsum(X) / M;
+    Y = wsum(Y) / M;
+    Z = wsum(Z) / M;
+  } else {  // massless bodies only: geometric centre
+    X = wsum(gx) / (e - s);
+    Y = wsum(gy) / (e - s);
+    Z = wsum(gz) / (e - s);
+  }
+  double q[6] = {0, 0, 0, 0, 0, 0}, rad = 0;
+  for (int k = s + lane; k < e; k += 64) {
+    const float4 b = sp[k];
+    const double dx = b.x - X, dy = b.y - Y, dz = b.z - Z, d2 = dx * dx + dy * dy + dz * dz;
+    q[0] += b.w * (3 * dx * dx - d2);
+    q[1] += b.w * (3 * dy * dy - d2);
+    q[2] += b.w * (3 * dz * dz - d2);
+    q[3] += b.w * 3 * dx * dy;
+    q[4] += b.w * 3 * dx * dz;
+    q[5] += b.w * 3 * dy * dz;
+    rad = fmax(rad, sqrt(d2));
+  }
+  for (int j = 0; j < 6; j++) q[j] = wsum(q[j]);
+  rad = wmax(rad);
+  if (lane < 2) {
+    float *r = (lane == 0 ? crec + (size_t)c * kCellRec : crec_ne + (size_t)t * kCellRec);
+    r[0] = (float)X;
+    r[1] = (float)Y;
+    r[2] = (float)Z;
+    r[3] = (float)M;
+    for (int j = 0; j < 6; j++) r[4 + j] = (float)q[j];
+    r[10] = (float)rad;
+    r[11] = (float)(c % G);
+    r[12] = (float)((c / G) % G);
+    r[13] = (float)(c / (G * G));
+    r[14] = (float)(e - s);
+    r[15] = 0.f;
+  }
+}
+
+// ---- far expansion: per nonempty target cell, sum over cells beyond distance K ---------------
+// Softened monopole field a = -M r / rs^3 (rs^2 = r^2 + eps^2), its Jacobian
+// J_ab = M (3 r_a r_b - rs^2 d_ab) / rs^5 and Hessian
+// H_abc = 3 M [(d_ab r_c + d_ac r_b + d_bc r_a) / rs^5 - 5 r_a r_b r_c / rs^7], plus the
+// quadrupole field Q r / rs^5 - 5/2 (r.Q.r) r / rs^7; r = X_target - X_source.
+//
+// Work unit = (64 target cells, one lane each) x (kM2LSlice source cells); the four waves of a
+// workgroup take a quarter of the slice each, streaming source records through the scalar cache,
+// and are summed in LDS in wave order.  Partials per slice go to `part`, summed by k_sur_m2l_sum.
+constexpr int kM2LSlice = 1024;
+
+__global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int32_t *necount,
+                                                 int K, float eps2_s, int ncell, float *part) {
+  __shared__ float red[3][kFarRec - 1][64];
+  const int nec = *necount;
+  const int ntg = (nec + 63) / 64, nsl = (nec + kM2LSlice - 1) / kM2LSlice;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const float eps2 = to_vgpr(eps2_s);
+  const float Kf = (float)K + 0.5f;
+  cf4p src4 = (cf4p)crec_ne;
+  for (int u = blockIdx.x; u < ntg * nsl; u += gridDim.x) {
+    const int tg = u % ntg, sl = u / ntg;
+    const int t = tg * 64 + lane;
+    const bool valid = t < nec;
+    const float *tr = crec_ne + (size_t)(valid ? t : nec - 1) * kCellRec;
+    const float X = tr[0], Y = tr[1], Z = tr[2], ix = tr[11], iy = tr[12], iz = tr[13];
+    float o[kFarRec - 1];
+#pragma unroll
+    for (int q = 0; q < kFarRec - 1; q++) o[q] = 0.f;
+    const int per = kM2LSlice / 4;
+    const int s0 = sl * kM2LSlice + wave * per, s1 = min(s0 + per, nec);
+#pragma unroll 2
+    for (int s = s0; s < s1; s++) {
+      const f4v r0 = src4[(size_t)s * 4 + 0], r1 = src4[(size_t)s * 4 + 1],
+                r2v = src4[(size_t)s * 4 + 2], r3 = src4[(size_t)s * 4 + 3];
+      // r0 = COM, M; r1 = Qxx Qyy Qzz Qxy; r2 = Qxz Qyz radius ix; r3 = iy iz n 0
+      const float cheb = fmaxf(fabsf(r2v.w - ix), fmaxf(fabsf(r3.x - iy), fabsf(r3.y - iz)));
+      const float rx = X - r0.x, ry = Y - r0.y, rz = Z - r0.z;
+      float rr = __builtin_fmaf(rx, rx, eps2);
+      rr = __builtin_fmaf(ry, ry, rr);
+      rr = __builtin_fmaf(rz, rz, rr);
+      float inv = __builtin_amdgcn_rsqf(rr);
+      inv = cheb > Kf ? inv : 0.f;
+      const float inv2 = inv * inv, M = r0.w;
+      const float m3 = M * inv2 * inv, m5 = m3 * inv2, m7 = m5 * inv2;
+      o[0] -= m3 * rx;
+      o[1] -= m3 * ry;
+      o[2] -= m3 * rz;
+      // quadrupole field
+      const float qx = r1.x * rx + r1.w * ry + r2v.x * rz;
+      const float qy = r1.w * rx + r1.y * ry + r2v.y * rz;
+      const float qz = r2v.x * rx + r2v.y * ry + r1.z * rz;
+      const float rqr = qx * rx + qy * ry + qz * rz;
+      const float i5 = inv2 * inv2 * inv, i7 = i5 * inv2;
+      const float c7 = 2.5f * rqr * i7;
+      o[0] += qx * i5 - c7 * rx;
+      o[1] += qy * i5 - c7 * ry;
+      o[2] += qz * i5 - c7 * rz;
+      // Jacobian of the monopole field (xx yy zz xy xz yz)
+      const float t3 = 3.f * m5;
+      o[3] += t3 * rx * rx - m3;
+      o[4] += t3 * ry * ry - m3;
+      o[5] += t3 * rz * rz - m3;
+      o[6] += t3 * rx * ry;
+      o[7] += t3 * rx * rz;
+      o[8] += t3 * ry * rz;
+      // Hessian of the monopole field (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz)
+      const float f5 = 15.f * m7;
+      const float ax5 = t3 * rx, ay5 = t3 * ry, az5 = t3 * rz;
+      o[9] += 3.f * ax5 - f5 * rx * rx * rx;
+      o[10] += ay5 - f5 * rx * rx * ry;
+      o[11] += az5 - f5 * rx * rx * rz;
+      o[12] += ax5 - f5 * rx * ry * ry;
+      o[13] += -f5 * rx * ry * rz;
+      o[14] += ax5 - f5 * rx * rz * rz;
+      o[15] += 3.f * ay5 - f5 * ry * ry * ry;
+      o[16] += az5 - f5 * ry * ry * rz;
+      o[17] += ay5 - f5 * ry * rz * rz;
+      o[18] += 3.f * az5 - f5 * rz * rz * rz;
+    }
+    if (wave > 0)
+      for (int q = 0; q < kFarRec - 1; q++) red[wave - 1][q][lane] = o[q];
+    __syncthreads();
+    if (wave == 0 && valid) {
+      float *dst = part + ((size_t)sl * ncell + t) * kFarRec;
+      for (int q = 0; q < kFarRec - 1; q++)
+        dst[q] = ((o[q] + red[0][q][lane]) + red[1][q][lane]) + red[2][q][lane];
+      dst[kFarRec - 1] = 0.f;
+    }
+    __syncthreads();
+  }
+}
+
+// fexp[cell] = sum of the slice partials in slice order (one thread per record word).
+__global__ void k_sur_m2l_sum(const float *part, const int32_t *nelist, const int32_t *necount,
+                              int ncell, float *fexp) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nec = *necount;
+  if (g >= nec * kFarRec) return;
+  const int t = g / kFarRec, q = g % kFarRec;
+  const int nsl = (nec + kM2LSlice - 1) / kM2LSlice;
+  float s = 0.f;
+  for (int sl = 0; sl < nsl; sl++) s += part[(size_t)sl * ncell * kFarRec + g];
+  fexp[(size_t)nelist[t] * kFarRec + q] = s;
+}
+
+// ---- near field, multipole shell, Taylor step and features per body ------------------------
+__device__ __forceinline__ void quad_field(f4v r0, f4v r1, f4v r2v, float px, float py,
+                                           float pz, float eps2, float &ax, float &ay,
+                                           float &az) {
+  const float rx = px - r0.x, ry = py - r0.y, rz = pz - r0.z;
+  float r2 = __builtin_fmaf(rx, rx, eps2);
+  r2 = __builtin_fmaf(ry, ry, r2);
+  r2 = __builtin_fmaf(rz, rz, r2);
+  const float inv = __builtin_amdgcn_rsqf(r2);
+  const float inv2 = inv * inv, m3 = r0.w * inv2 * inv;
+  const float qx = r1.x * rx + r1.w * ry + r2v.x * rz;
+  const float qy = r1.w * rx + r1.y * ry + r2v.y * rz;
+  const float qz = r2v.x * rx + r2v.y * ry + r1.z * rz;
+  const float rqr = qx * rx + qy * ry + qz * rz;
+  const float i5 = inv2 * inv2 * inv, c7 = 2.5f * rqr * i5 * inv2;
+  ax += -m3 * rx + qx * i5 - c7 * rx;
+  ay += -m3 * ry + qy * i5 - c7 * ry;
+  az += -m3 * rz + qz * i5 - c7 * rz;
+}
+
+__device__ __forceinline__ void near_pair(f4v b, float4 me, float eps2, float &px, float &py,
+                                          float &pz) {
+  const float dx = b.x - me.x, dy = b.y - me.y, dz = b.z - me.z;
+  float r2 = __builtin_fmaf(dx, dx, eps2);
+  r2 = __builtin_fmaf(dy, dy, r2);
+  r2 = __builtin_fmaf(dz, dz, r2);
+  const float inv = __builtin_amdgcn_rsqf(r2);
+  const float s = b.w * inv * inv * inv;
+  px = __builtin_fmaf(dx, s, px);
+  py = __builtin_fmaf(dy, s, py);
+  pz = __builtin_fmaf(dz, s, pz);
+}
+
+struct NearArgs {
+  const float4 *sp;
+  const int32_t *cstart, *ccount;
+  const int2 *items;
+  const int32_t *nitems;
+  const float *crec, *fexp;
+  const float *mu, *sd;
+  float *feat;
+  float4 *base;
+  int G, K, write_feat;
+  float eps2, r0;
+  double *near_pairs;
+};
+
+// One workgroup (4 waves) per work item = up to 64 bodies of one cell, one lane each.  The waves
+// split the cell's neighbourhood: 64-source chunks of the nine contiguous rows of adjacent cells
+// (round robin over the chunks) and the shell cells (round robin), streaming both through the
+// scalar cache; partial sums meet in LDS in wave order, and wave 0 adds the Taylor step of the
+// far expansion and writes the base acceleration and the MLP features.
+__global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
+  __shared__ double red[3][3][64];
+  const int item = blockIdx.x;
+  if (item >= *a.nitems) return;  // uniform per workgroup
+  const int G = a.G;
+  const int2 it = a.items[item];
+  const int c = it.x;
+  const int cx = c % G, cy = (c / G) % G, cz = c / (G * G);
+  cip cs = (cip)a.cstart;
+  cip cc = (cip)a.ccount;
+  cf4p sp4 = (cf4p)a.sp;
+  cf4p cr4 = (cf4p)a.crec;
+  const int c_begin = cs[c], c_end = cs[c + 1];
+  const int t_begin = c_begin + it.y * 64, t_end = min(c_end, t_begin + 64);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int k = t_begin + lane;
+  const bool tv = k < t_end;
+  const float4 me = a.sp[tv ? k : t_begin];
+  const float eps2 = to_vgpr(a.eps2);
+  double ax = 0, ay = 0, az = 0;
+  int chunk = 0, nsrc = 0;
+  const int xl = max(cx - 1, 0), xh = min(cx + 1, G - 1);
+  for (int z = max(cz - 1, 0); z <= min(cz + 1, G - 1); z++)
+    for (int y = max(cy - 1, 0); y <= min(cy + 1, G - 1); y++) {
+      const int row = G * (y + G * z);
+      const int r0i = cs[xl + row], r1i = cs[xh + row + 1];
+      nsrc += r1i - r0i;
+      for (int s0 = r0i; s0 < r1i; s0 += 64, chunk++) {
+        if ((chunk & 3) != wave) continue;
+        const int cnt = min(64, r1i - s0);
+        float px = 0, py = 0, pz = 0;
+        int j = 0;
+        for (; j + 4 <= cnt; j += 4) {  // four sources per scalar fetch
+          const f4v b0 = sp4[s0 + j], b1 = sp4[s0 + j + 1], b2 = sp4[s0 + j + 2],
+                    b3 = sp4[s0 + j + 3];
+          near_pair(b0, me, eps2, px, py, pz);
+          near_pair(b1, me, eps2, px, py, pz);
+          near_pair(b2, me, eps2, px, py, pz);
+          near_pair(b3, me, eps2, px, py, pz);
+        }
+        for (; j < cnt; j++) near_pair(sp4[s0 + j], me, eps2, px, py, pz);
+        ax += px;
+        ay += py;
+        az += pz;
+      }
+    }
+  // multipole shell, Chebyshev distance 2..K (cells dealt round robin to the waves)
+  {
+    const int K = a.K;
+    float sx = 0, sy = 0, sz = 0;
+    int q = 0;
+    for (int z = max(cz - K, 0); z <= min(cz + K, G - 1); z++)
+      for (int y = max(cy - K, 0); y <= min(cy + K, G - 1); y++)
+        for (int x = max(cx - K, 0); x <= min(cx + K, G - 1); x++) {
+          const int ch = max(abs(x - cx), max(abs(y - cy), abs(z - cz)));
+          if (ch < 2) continue;
+          const bool mine = (q & 3) == wave;
+          q++;
+          if (!mine) continue;
+          const int s = x + G * (y + G * z);
+          if (cc[s] == 0) continue;
+          quad_field(cr4[(size_t)s * 4], cr4[(size_t)s * 4 + 1], cr4[(size_t)s * 4 + 2], me.x,
+                     me.y, me.z, eps2, sx, sy, sz);
+        }
+    ax += sx;
+    ay += sy;
+    az += sz;
+  }
+  if (wave > 0) {
+    red[wave - 1][0][lane] = ax;
+    red[wave - 1][1][lane] = ay;
+    red[wave - 1][2][lane] = az;
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  if (lane == 0) atomicAdd(a.near_pairs, (double)(t_end - t_begin) * (double)nsrc);
+  if (!tv) return;
+  ax = ((ax + red[0][0][lane]) + red[1][0][lane]) + red[2][0][lane];
+  ay = ((ay + red[0][1][lane]) + red[1][1][lane]) + red[2][1][lane];
+  az = ((az + red[0][2][lane]) + red[1][2][lane]) + red[2][2][lane];
+  // second-order Taylor step of the far expansion from the cell's centre of mass
+  cfp cr = (cfp)(a.crec + (size_t)c * kCellRec);
+  cfp fe = (cfp)(a.fexp + (size_t)c * kFarRec);
+  const float dx = me.x - cr[0], dy = me.y - cr[1], dz = me.z - cr[2];
+  cfp J = fe + 3;
+  cfp H = fe + 9;
+  float tx = fe[0] + J[0] * dx + J[3] * dy + J[4] * dz;
+  float ty = fe[1] + J[3] * dx + J[1] * dy + J[5] * dz;
+  float tz = fe[2] + J[4] * dx + J[5] * dy + J[2] * dz;
+  // 1/2 H_abc d_b d_c with H index (xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz)
+  tx += 0.5f * (H[0] * dx * dx + H[3] * dy * dy + H[5] * dz * dz +
+                2.f * (H[1] * dx * dy + H[2] * dx * dz + H[4] * dy * dz));
+  ty += 0.5f * (H[1] * dx * dx + H[6] * dy * dy + H[8] * dz * dz +
+                2.f * (H[3] * dx * dy + H[4] * dx * dz + H[7] * dy * dz));
+  tz += 0.5f * (H[2] * dx * dx + H[7] * dy * dy + H[9] * dz * dz +
+                2.f * (H[4] * dx * dy + H[5] * dx * dz + H[8] * dy * dz));
+  ax += tx;
+  ay += ty;
+  az += tz;
+  // scale-free features
+  const float sc = fmaxf(cr[10], 1e-6f * a.r0);
+  const float na = sqrtf(fe[0] * fe[0] + fe[1] * fe[1] + fe[2] * fe[2]);
+  float nj = 0, nh = 0;
+  for (int q = 0; q < 6; q++) nj += J[q] * J[q] * (q < 3 ? 1.f : 2.f);
+  const float hw[10] = {1, 3, 3, 3, 6, 3, 1, 3, 3, 1};
+  for (int q = 0; q < 10; q++) nh += hw[q] * H[q] * H[q];
+  const float A = na + sqrtf(nj) * sc + sqrtf(nh) * sc * sc + 1e-30f;
+  a.base[k] = make_float4((float)ax, (float)ay, (float)az, A);
+  if (a.write_feat) {
+    float f[kFeat];
+    const float ux = dx / sc, uy = dy / sc, uz = dz / sc;
+    f[0] = ux; f[1] = uy; f[2] = uz;
+    f[3] = ux * ux; f[4] = uy * uy; f[5] = uz * uz; f[6] = ux * uy; f[7] = ux * uz;
+    f[8] = uy * uz;
+    const float iA = 1.f / A;
+    f[9] = fe[0] * iA; f[10] = fe[1] * iA; f[11] = fe[2] * iA;
+    for (int q = 0; q < 6; q++) f[12 + q] = J[q] * sc * iA;
+    for (int q = 0; q < 10; q++) f[18 + q] = H[q] * sc * sc * iA;
+    f[28] = logf(sc / a.r0);
+    f[29] = logf((float)(c_end - c_begin));
+    float *o = a.feat + (size_t)k * kFeat;
+    for (int q = 0; q < kFeat; q++) o[q] = (f[q] - a.mu[q]) / a.sd[q];
+  }
+}
+
+// Scatter the surrogate's accelerations (base + learned residual) to the accumulator slice 0.
+__global__ void k_sur_apply(const uint32_t *perm, const uint32_t *keys2, int n, int ncell,
+                            const float4 *base, const float *Y, int ldy, float out_scale,
+                            int use_mlp, double *acc, int npad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || (int)keys2[k] >= ncell) return;
+  const uint32_t i = perm[k];
+  const float4 b = base[k];
+  double ax = b.x, ay = b.y, az = b.z;
+  if (use_mlp) {
+    const double s = (double)b.w * out_scale;
+    ax += s * Y[(size_t)k * ldy + 0];
+    ay += s * Y[(size_t)k * ldy + 1];
+    az += s * Y[(size_t)k * ldy + 2];
+  }
+  acc[i] = ax;
+  acc[npad + i] = ay;
+  acc[2 * (size_t)npad + i] = az;
+}
+
+// ---- sampled exact check -------------------------------------------------------------------
+// One workgroup per sampled (sorted) body: FP32 direct sum over all active bodies, threads over
+// strided sources, FP64 tree reduction in fixed order.
+__global__ __launch_bounds__(256) void k_sur_exact(const float4 *sp, const int32_t *cstart,
+                                                   int ncell, const int32_t *sidx, float eps2_s,
+                                                   double *out) {
+  __shared__ double sh[3][256];
+  const int nact = cstart[ncell];
+  const int k = sidx[blockIdx.x];
+  const float eps2 = to_vgpr(eps2_s);
+  float px = 0, py = 0, pz = 0;
+  if (k < nact) {
+    const float4 me = sp[k];
+    for (int j = threadIdx.x; j < nact; j += 256) {
+      const float4 b = sp[j];
+      const float dx = b.x - me.x, dy = b.y - me.y, dz = b.z - me.z;
+      float r2 = __builtin_fmaf(dx, dx, eps2);
+      r2 = __builtin_fmaf(dy, dy, r2);
+      r2 = __builtin_fmaf(dz, dz, r2);
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float s = b.w * inv * inv * inv;
+      px = __builtin_fmaf(dx, s, px);
+      py = __builtin_fmaf(dy, s, py);
+      pz = __builtin_fmaf(dz, s, pz);
+    }
+  }
+  sh[0][threadIdx.x] = px;
+  sh[1][threadIdx.x] = py;
+  sh[2][threadIdx.x] = pz;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < 3; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; q++) out[(size_t)blockIdx.x * 3 + q] = sh[q][0];
+}
+
+// Relative error of the surrogate over the samples; raises the fallback flag above tol.
+__global__ void k_sur_check(const int32_t *sidx, int S, const uint32_t *perm,
+                            const int32_t *cstart, int ncell, const double *exact,
+                            const double *acc, int npad, double tol, uint32_t *flag,
+                            double *err) {
+  if (threadIdx.x != 0) return;
+  const int nact = cstart[ncell];
+  double num = 0, den = 0;
+  for (int s = 0; s < S; s++) {
+    const int k = sidx[s];
+    if (k >= nact) continue;
+    const uint32_t i = perm[k];
+    for (int q = 0; q < 3; q++) {
+      const double e = exact[(size_t)s * 3 + q];
+      const double d = acc[(size_t)q * npad + i] - e;
+      num += d * d;
+      den += e * e;
+    }
+  }
+  const double r = den > 0 ? sqrt(num / den) : 0.0;
+  *err = r;
+  *flag = r > tol ? 1u : 0u;
+}
+
+// After a fallback force launch: fold the plain kernel's slices into slice 0 (flag set only).
+__global__ void k_sur_fold(double *acc, int split, int npad, int n, const uint32_t *flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !*flag) return;
+  for (int c = 0; c < 3; c++) {
+    double s = 0;
+    for (int q = 0; q < split; q++) s += acc[((size_t)q * 3 + c) * npad + i];
+    acc[(size_t)c * npad + i] = s;
+  }
+}
+
+__global__ void k_sur_setflag(uint32_t *flag, uint32_t v) {
+  if (threadIdx.x == 0) *flag = v;
+}
+
+// Training rows: features of sampled bodies and their normalised residual
+// (a_exact - base) / A, with a_exact folded from the plain kernel's slices.
+__global__ void k_sur_label(const int32_t *sidx, int S, const uint32_t *perm,
+                            const uint32_t *keys2, int ncell, const float4 *base,
+                            const float *feat, const double *acc, int split, int npad,
+                            float *tx, float *ty) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int k = sidx[s];
+  const uint32_t i = perm[k];
+  const float4 b = base[k];
+  double ex[3] = {0, 0, 0};
+  for (int q = 0; q < split; q++)
+    for (int c = 0; c < 3; c++) ex[c] += acc[((size_t)q * 3 + c) * npad + i];
+  ty[(size_t)s * 3 + 0] = (float)((ex[0] - b.x) / b.w);
+  ty[(size_t)s * 3 + 1] = (float)((ex[1] - b.y) / b.w);
+  ty[(size_t)s * 3 + 2] = (float)((ex[2] - b.z) / b.w);
+  for (int q = 0; q < kFeat; q++) tx[(size_t)s * kFeat + q] = feat[(size_t)k * kFeat + q];
+  (void)keys2;
+  (void)ncell;
+}
+
+__global__ void k_sur_gather_rows(const float *src, const int32_t *rows, int nrows, int width,
+                                  float *dst) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)nrows * width) return;
+  const int r = (int)(q / width), c = (int)(q % width);
+  dst[q] = src[(size_t)rows[r] * width + c];
+}
+
+inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace nb
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+namespace nb {
+
+__global__ void k_sur_samples(int32_t *sidx, int S, const int32_t *cstart, int ncell) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int nact = cstart[ncell];
+  sidx[s] = nact > 0 ? (int)(((int64_t)s * nact) / S) : 0;
+}
+
+__global__ void k_sur_count_fallback(const uint32_t *flag, DevCounters *ctr) {
+  if (threadIdx.x == 0 && *flag) atomicAdd((unsigned long long *)&ctr->sur_fallbacks, 1ull);
+}
+
+template <class T>
+static hipError_t salloc(T **p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  return hipMalloc((void **)p, (count ? count : 1) * sizeof(T));
+}
+
+void sur_free(Sur *s) {
+  if (!s) return;
+  void *ptrs[] = {s->keys, s->keys2, s->perm_in, s->perm, s->sort_tmp, s->cstart, s->ccount,
+                  s->nelist, s->necount, s->sp, s->crec, s->crec_ne, s->fexp, s->part,
+                  s->items, s->nitems, s->feat, s->base, s->center,
+                  s->red, s->mu, s->sd, s->tx, s->ty, s->sidx, s->sexact, s->serr, s->flag,
+                  s->mu0, s->sd1, s->near_pairs};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  if (s->mlp) nb_mlp_destroy(s->mlp);
+  delete s;
+}
+
+static int sur_ensure_bodies(Ctx *c) {
+  Sur *s = c->sur;
+  const int64_t need_items = (int64_t)s->ncell + c->cap / 64 + 2;
+  if (s->items_cap < need_items) {
+    NB_HIP(c, salloc(&s->items, (size_t)need_items));
+    s->items_cap = need_items;
+  }
+  if (s->cap >= c->cap) return NB_OK;
+  const size_t n = (size_t)c->cap;
+  NB_HIP(c, salloc(&s->keys, n));
+  NB_HIP(c, salloc(&s->keys2, n));
+  NB_HIP(c, salloc(&s->perm_in, n));
+  NB_HIP(c, salloc(&s->perm, n));
+  NB_HIP(c, salloc(&s->sp, n));
+  NB_HIP(c, salloc(&s->base, n));
+  NB_HIP(c, salloc(&s->feat, n * kFeat));
+  size_t bytes = 0;
+  NB_HIP(c, rocprim::radix_sort_pairs(nullptr, bytes, s->keys, s->keys2, s->perm_in, s->perm,
+                                      (size_t)n, 0u, 16u, c->stream));
+  void *tmp = nullptr;
+  NB_HIP(c, hipMalloc(&tmp, bytes ? bytes : 1));
+  if (s->sort_tmp) (void)hipFree(s->sort_tmp);
+  s->sort_tmp = tmp;
+  s->sort_bytes = bytes;
+  s->cap = c->cap;
+  return NB_OK;
+}
+
+static int sur_ensure_samples(Ctx *c, int S) {
+  Sur *s = c->sur;
+  if (S <= s->S) return NB_OK;
+  NB_HIP(c, salloc(&s->sidx, (size_t)S));
+  NB_HIP(c, salloc(&s->sexact, (size_t)S * 3));
+  s->S = S;
+  return NB_OK;
+}
+
+// Binning, moments, far expansion, near field (+ features).  apply: scatter into acc slice 0.
+static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool use_mlp) {
+  Sur *s = c->sur;
+  hipStream_t st = c->stream;
+  const int n = (int)c->n, G = s->G, ncell = s->ncell;
+  if (n == 0) return NB_OK;
+  int rc = sur_ensure_bodies(c);
+  if (rc) return rc;
+  const int nbk = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(k_sur_com_part, dim3(nbk), dim3(256), 0, st, c->x, c->y, c->z, c->m,
+                     c->flags, n, s->red);
+  hipLaunchKernelGGL(k_sur_com_final, dim3(1), dim3(256), 0, st, s->red, nbk, s->center);
+  hipLaunchKernelGGL(k_sur_keys, dim3(nb256(n)), dim3(256), 0, st, c->x, c->y, c->z, c->flags,
+                     n, s->center, s->r0, G, s->keys, s->perm_in);
+  NB_HIP(c, hipGetLastError());
+  size_t bytes = s->sort_bytes;
+  NB_HIP(c, rocprim::radix_sort_pairs(s->sort_tmp, bytes, s->keys, s->keys2, s->perm_in, s->perm,
+                                      (size_t)n, 0u, 16u, st));
+  hipLaunchKernelGGL(k_sur_bounds, dim3(nb256(ncell + 1)), dim3(256), 0, st, s->keys2, n, ncell,
+                     s->cstart);
+  hipLaunchKernelGGL(k_sur_scan, dim3(1), dim3(1024), 0, st, s->cstart, ncell, s->ccount,
+                     s->nelist, s->necount, s->items, s->nitems);
+  hipLaunchKernelGGL(k_sur_gather, dim3(nb256(n)), dim3(256), 0, st, s->perm, s->keys2, n, ncell,
+                     c->x, c->y, c->z, c->m, s->center, s->sp);
+  hipLaunchKernelGGL(k_sur_moments, dim3((unsigned)ncell), dim3(64), 0, st, s->sp, s->cstart,
+                     s->nelist, s->necount, G, s->crec, s->crec_ne);
+  const float eps2 = (float)(c->p.eps * c->p.eps);
+  hipLaunchKernelGGL(k_sur_m2l, dim3(2048), dim3(256), 0, st, s->crec_ne, s->necount, s->K, eps2,
+                     ncell, s->part);
+  hipLaunchKernelGGL(k_sur_m2l_sum, dim3(nb256((int64_t)ncell * kFarRec)), dim3(256), 0, st,
+                     s->part, s->nelist,
+                     s->necount, ncell, s->fexp);
+  NB_HIP(c, hipGetLastError());
+  NearArgs na;
+  na.sp = s->sp;
+  na.cstart = s->cstart;
+  na.ccount = s->ccount;
+  na.items = s->items;
+  na.nitems = s->nitems;
+  na.crec = s->crec;
+  na.fexp = s->fexp;
+  na.mu = feat_norm ? s->mu : s->mu0;
+  na.sd = feat_norm ? s->sd : s->sd1;
+  na.feat = s->feat;
+  na.base = s->base;
+  na.G = G;
+  na.K = s->K;
+  na.write_feat = (feat_raw || feat_norm) ? 1 : 0;
+  na.eps2 = eps2;
+  na.r0 = (float)s->r0;
+  na.near_pairs = s->near_pairs;
+  const unsigned item_ub = (unsigned)(ncell + (n + 63) / 64 + 1);
+  hipLaunchKernelGGL(k_sur_near, dim3(item_ub), dim3(256), 0, st, na);
+  NB_HIP(c, hipGetLastError());
+  if (!apply) return NB_OK;
+  const float *Y = nullptr;
+  int ldy = 0;
+  const bool mlp_on = use_mlp && s->trained && s->mlp;
+  if (mlp_on) {
+    rc = mlp_forward_device(s->mlp, s->feat, n, &Y, &ldy, st);
+    if (rc) return fail_msg(c, rc, "surrogate MLP forward failed");
+  }
+  hipLaunchKernelGGL(k_sur_apply, dim3(nb256(n)), dim3(256), 0, st, s->perm, s->keys2, n, ncell,
+                     s->base, Y, ldy, s->out_scale, mlp_on ? 1 : 0, c->acc, (int)c->npad);
+  NB_HIP(c, hipGetLastError());
+  return NB_OK;
+}
+
+// The force phase of one cycle when the surrogate is on.
+int sur_force_phase(Ctx *c, bool timed, int ev_slot) {
+  Sur *s = c->sur;
+  hipStream_t st = c->stream;
+  NB_HIP(c, launch_zero_counters(c->ctr, 0, st));
+  if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot], st));
+  int rc = sur_pipeline(c, false, s->p.use_mlp && s->trained, true, s->p.use_mlp != 0);
+  if (rc) return rc;
+  if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], st));
+  if (s->p.check_every > 0 && s->steps % s->p.check_every == 0 && c->n > 0) {
+    const int S = s->p.check_samples;
+    rc = sur_ensure_samples(c, S);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sur_samples, dim3(nb256(S)), dim3(256), 0, st, s->sidx, S, s->cstart,
+                       s->ncell);
+    hipLaunchKernelGGL(k_sur_exact, dim3((unsigned)S), dim3(256), 0, st, s->sp, s->cstart,
+                       s->ncell, s->sidx, (float)(c->p.eps * c->p.eps), s->sexact);
+    hipLaunchKernelGGL(k_sur_check, dim3(1), dim3(64), 0, st, s->sidx, S, s->perm, s->cstart,
+                       s->ncell, s->sexact, c->acc, (int)c->npad, s->p.tol, s->flag, s->serr);
+    hipLaunchKernelGGL(k_sur_count_fallback, dim3(1), dim3(64), 0, st, s->flag, c->ctr);
+    NB_HIP(c, hipGetLastError());
+    // direct-sum redo of this cycle's force when the flag is up (the kernel exits otherwise)
+    ForceArgs fa = make_force_args(c);
+    fa.run_flag = s->flag;
+    NB_HIP(c, launch_force_f32(c->shape, fa, st));
+    hipLaunchKernelGGL(k_sur_fold, dim3(nb256(c->n)), dim3(256), 0, st, c->acc, c->shape.jsplit,
+                       (int)c->npad, (int)c->n, s->flag);
+    NB_HIP(c, hipGetLastError());
+    s->checks++;
+  }
+  s->steps++;
+  return NB_OK;
+}
+
+}  // namespace nb
+
+using namespace nb;
+
+extern "C" {
+
+int nb_sur_params_init(nb_sur_params *p) {
+  if (!p) return NB_E_INVALID;
+  memset(p, 0, sizeof *p);
+  p->grid = 32;
+  p->shell = 2;
+  p->r0 = 0.0;
+  p->use_mlp = 1;
+  p->check_every = 1;
+  p->check_samples = 256;
+  p->tol = 1e-3;
+  p->seed = 12345;
+  return NB_OK;
+}
+
+int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
+  if (!c || !p) return NB_E_INVALID;
+  if (!c->loaded) return fail_msg(c, NB_E_STATE, "nb_sur_enable: load bodies first");
+  if (c->p.precision != NB_PREC_FP32 || !(c->p.eps > 0.0))
+    return fail_msg(c, NB_E_UNSUPPORTED, "the surrogate needs the FP32 path and eps > 0");
+  if (c->p.collisions || c->p.overlap_cutoff)
+    return fail_msg(c, NB_E_UNSUPPORTED,
+                    "the surrogate path has no collision detection or overlap cutoff");
+  if (p->grid < 4 || p->grid > kMaxGrid || p->shell < 1 || p->shell > 6 || p->check_every < 0 ||
+      (p->check_every > 0 && (p->check_samples < 1 || p->check_samples > 65536)) ||
+      !(p->tol > 0) || p->r0 < 0)
+    return fail_msg(c, NB_E_INVALID, "nb_sur_enable: bad parameters");
+  NB_HIP(c, hipSetDevice(c->device));
+  Sur *s = c->sur;
+  const bool fresh = !s;
+  if (fresh) {
+    s = new (std::nothrow) Sur();
+    if (!s) return fail_msg(c, NB_E_NOMEM, "out of host memory");
+    c->sur = s;
+  }
+  const bool grid_changed = fresh || s->G != p->grid;
+  s->p = *p;
+  s->G = p->grid;
+  s->ncell = p->grid * p->grid * p->grid;
+  s->K = p->shell;
+  if (grid_changed) {
+    const size_t nc = (size_t)s->ncell;
+    NB_HIP(c, salloc(&s->cstart, nc + 1));
+    NB_HIP(c, salloc(&s->ccount, nc));
+    NB_HIP(c, salloc(&s->nelist, nc));
+    NB_HIP(c, salloc(&s->necount, 1));
+    NB_HIP(c, salloc(&s->crec, nc * kCellRec));
+    NB_HIP(c, salloc(&s->crec_ne, nc * kCellRec));
+    NB_HIP(c, salloc(&s->fexp, nc * kFarRec));
+    NB_HIP(c, salloc(&s->part, nc * kFarRec * (size_t)((nc + kM2LSlice - 1) / kM2LSlice)));
+    s->items_cap = 0;  // re-sized for the new grid by sur_ensure_bodies
+  }
+  if (fresh) {
+    NB_HIP(c, salloc(&s->center, 4));
+    NB_HIP(c, salloc(&s->red, 1024 * 4));
+    NB_HIP(c, salloc(&s->mu, kFeat));
+    NB_HIP(c, salloc(&s->sd, kFeat));
+    NB_HIP(c, salloc(&s->mu0, kFeat));
+    NB_HIP(c, salloc(&s->sd1, kFeat));
+    NB_HIP(c, salloc(&s->serr, 1));
+    NB_HIP(c, salloc(&s->flag, 1));
+    NB_HIP(c, salloc(&s->near_pairs, 1));
+    NB_HIP(c, salloc(&s->nitems, 1));
+    std::vector<float> z(kFeat, 0.f), o(kFeat, 1.f);
+    NB_HIP(c, hipMemcpy(s->mu0, z.data(), kFeat * 4, hipMemcpyHostToDevice));
+    NB_HIP(c, hipMemcpy(s->sd1, o.data(), kFeat * 4, hipMemcpyHostToDevice));
+    NB_HIP(c, hipMemcpy(s->mu, z.data(), kFeat * 4, hipMemcpyHostToDevice));
+    NB_HIP(c, hipMemcpy(s->sd, o.data(), kFeat * 4, hipMemcpyHostToDevice));
+    NB_HIP(c, hipMemset(s->serr, 0, sizeof(double)));
+    NB_HIP(c, hipMemset(s->flag, 0, sizeof(uint32_t)));
+    NB_HIP(c, hipMemset(s->near_pairs, 0, sizeof(double)));
+  }
+  // stretch radius: 1.5 x the half-mass radius about the centre of mass
+  if (p->r0 > 0) {
+    s->r0 = p->r0;
+  } else {
+    const size_t n = (size_t)c->n;
+    std::vector<double> hx(n), hy(n), hz(n), hm(n);
+    std::vector<uint8_t> hf(n);
+    NB_HIP(c, hipMemcpy(hx.data(), c->x, n * 8, hipMemcpyDeviceToHost));
+    NB_HIP(c, hipMemcpy(hy.data(), c->y, n * 8, hipMemcpyDeviceToHost));
+    NB_HIP(c, hipMemcpy(hz.data(), c->z, n * 8, hipMemcpyDeviceToHost));
+    NB_HIP(c, hipMemcpy(hm.data(), c->m, n * 8, hipMemcpyDeviceToHost));
+    NB_HIP(c, hipMemcpy(hf.data(), c->flags, n, hipMemcpyDeviceToHost));
+    double M = 0, X = 0, Y = 0, Z = 0;
+    for (size_t i = 0; i < n; i++)
+      if ((hf[i] & NB_F_EXISTS) && !(hf[i] & NB_F_FRAGMENTING)) {
+        M += hm[i];
+        X += hm[i] * hx[i];
+        Y += hm[i] * hy[i];
+        Z += hm[i] * hz[i];
+      }
+    if (M > 0) {
+      X /= M;
+      Y /= M;
+      Z /= M;
+    }
+    std::vector<std::pair<double, double>> rm;
+    for (size_t i = 0; i < n; i++)
+      if ((hf[i] & NB_F_EXISTS) && !(hf[i] & NB_F_FRAGMENTING))
+        rm.push_back({std::sqrt((hx[i] - X) * (hx[i] - X) + (hy[i] - Y) * (hy[i] - Y) +
+                                (hz[i] - Z) * (hz[i] - Z)),
+                      hm[i]});
+    std::sort(rm.begin(), rm.end());
+    double acc = 0, rh = 1.0;
+    for (auto &q : rm) {
+      acc += q.second;
+      if (acc >= 0.5 * M) {
+        rh = q.first;
+        break;
+      }
+    }
+    s->r0 = 1.5 * std::max(rh, 1e-30);
+  }
+  c->sur_on = true;
+  return sur_ensure_bodies(c);
+}
+
+int nb_sur_disable(nb_ctx *c) {
+  if (!c) return NB_E_INVALID;
+  c->sur_on = false;
+  return NB_OK;
+}
+
+// Append training rows (features and normalised far-field residual) for `samples` bodies of the
+// loaded state, labelled by the plain FP32 direct sum.
+int nb_sur_collect(nb_ctx *c, int64_t samples, uint64_t seed) {
+  if (!c) return NB_E_INVALID;
+  if (!c->sur_on) return fail_msg(c, NB_E_STATE, "nb_sur_collect: surrogate not enabled");
+  if (samples < 1 || samples > (1 << 24))
+    return fail_msg(c, NB_E_INVALID, "nb_sur_collect: samples out of range");
+  Sur *s = c->sur;
+  NB_HIP(c, hipSetDevice(c->device));
+  int rc = sur_pipeline(c, true, false, false, false);
+  if (rc) return rc;
+  ForceArgs fa = make_force_args(c);
+  fa.run_flag = nullptr;
+  NB_HIP(c, launch_force_f32(c->shape, fa, c->stream));
+  int32_t nact = 0;
+  NB_HIP(c, hipMemcpyAsync(&nact, s->cstart + s->ncell, 4, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
+  if (nact == 0) return NB_OK;
+  const int64_t S = std::min<int64_t>(samples, nact);
+  std::vector<int32_t> pick((size_t)nact);
+  for (int32_t q = 0; q < nact; q++) pick[q] = q;
+  std::mt19937_64 rng(seed);
+  for (int64_t q = 0; q < S; q++) {
+    std::uniform_int_distribution<int64_t> U(q, nact - 1);
+    std::swap(pick[(size_t)q], pick[(size_t)U(rng)]);
+  }
+  pick.resize((size_t)S);
+  std::sort(pick.begin(), pick.end());
+  if (s->trows + S > s->tcap) {
+    const int64_t ncap = std::max<int64_t>(2 * s->tcap, s->trows + S);
+    float *nx = nullptr, *ny = nullptr;
+    NB_HIP(c, hipMalloc((void **)&nx, (size_t)ncap * kFeat * 4));
+    NB_HIP(c, hipMalloc((void **)&ny, (size_t)ncap * 3 * 4));
+    if (s->trows) {
+      NB_HIP(c, hipMemcpy(nx, s->tx, (size_t)s->trows * kFeat * 4, hipMemcpyDeviceToDevice));
+      NB_HIP(c, hipMemcpy(ny, s->ty, (size_t)s->trows * 3 * 4, hipMemcpyDeviceToDevice));
+    }
+    if (s->tx) (void)hipFree(s->tx);
+    if (s->ty) (void)hipFree(s->ty);
+    s->tx = nx;
+    s->ty = ny;
+    s->tcap = ncap;
+  }
+  int32_t *dpick = nullptr;
+  NB_HIP(c, hipMalloc((void **)&dpick, (size_t)S * 4));
+  hipError_t e = hipMemcpyAsync(dpick, pick.data(), (size_t)S * 4, hipMemcpyHostToDevice,
+                                c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_sur_label, dim3(nb256(S)), dim3(256), 0, c->stream, dpick, (int)S,
+                       s->perm, s->keys2, s->ncell, s->base, s->feat, c->acc, c->shape.jsplit,
+                       (int)c->npad, s->tx + (size_t)s->trows * kFeat, s->ty + (size_t)s->trows * 3);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dpick);
+  NB_HIP(c, e);
+  s->trows += S;
+  return NB_OK;
+}
+
+int nb_sur_fit(nb_ctx *c, int32_t steps, int32_t batch, float lr, float *loss) {
+  if (!c) return NB_E_INVALID;
+  if (!c->sur_on || !c->sur) return fail_msg(c, NB_E_STATE, "nb_sur_fit: surrogate not enabled");
+  Sur *s = c->sur;
+  if (s->trows < batch || batch < 64 || batch % 64 || steps < 1)
+    return fail_msg(c, NB_E_INVALID, "nb_sur_fit: need at least one batch of collected rows");
+  NB_HIP(c, hipSetDevice(c->device));
+  const size_t R = (size_t)s->trows;
+  std::vector<float> X(R * kFeat), Y(R * 3);
+  NB_HIP(c, hipMemcpy(X.data(), s->tx, X.size() * 4, hipMemcpyDeviceToHost));
+  NB_HIP(c, hipMemcpy(Y.data(), s->ty, Y.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<double> mu(kFeat, 0.0), var(kFeat, 0.0);
+  for (size_t r = 0; r < R; r++)
+    for (int q = 0; q < kFeat; q++) mu[q] += X[r * kFeat + q];
+  for (int q = 0; q < kFeat; q++) mu[q] /= (double)R;
+  for (size_t r = 0; r < R; r++)
+    for (int q = 0; q < kFeat; q++) {
+      const double d = X[r * kFeat + q] - mu[q];
+      var[q] += d * d;
+    }
+  std::vector<float> fmu(kFeat), fsd(kFeat);
+  for (int q = 0; q < kFeat; q++) {
+    fmu[q] = (float)mu[q];
+    fsd[q] = (float)std::max(std::sqrt(var[q] / (double)R), 1e-12);
+  }
+  double ys = 0;
+  for (float v : Y) ys += (double)v * v;
+  const float oscale = (float)std::max(std::sqrt(ys / (double)Y.size()), 1e-30);
+  // shuffle rows once (seeded), normalise inputs and targets
+  std::vector<size_t> ord(R);
+  for (size_t r = 0; r < R; r++) ord[r] = r;
+  std::mt19937_64 rng(s->p.seed);
+  std::shuffle(ord.begin(), ord.end(), rng);
+  std::vector<float> Xn(R * kFeat), Yn(R * 3);
+  for (size_t r = 0; r < R; r++) {
+    const size_t o = ord[r];
+    for (int q = 0; q < kFeat; q++) Xn[r * kFeat + q] = (X[o * kFeat + q] - fmu[q]) / fsd[q];
+    for (int q = 0; q < 3; q++) Yn[r * 3 + q] = Y[o * 3 + q] / oscale;
+  }
+  if (!s->mlp) {
+    int rc = nb_mlp_create(c->device, kFeat, 128, 3, 3, s->p.seed, &s->mlp);
+    if (rc) return fail_msg(c, rc, "nb_sur_fit: could not create the MLP");
+  }
+  const int64_t rows = (int64_t)(R / batch) * batch;
+  float l = 0;
+  int rc = nb_mlp_train(s->mlp, Xn.data(), Yn.data(), rows, steps, batch, lr, &l);
+  if (rc) return fail_msg(c, rc, nb_mlp_last_error(s->mlp));
+  NB_HIP(c, hipMemcpy(s->mu, fmu.data(), kFeat * 4, hipMemcpyHostToDevice));
+  NB_HIP(c, hipMemcpy(s->sd, fsd.data(), kFeat * 4, hipMemcpyHostToDevice));
+  s->out_scale = oscale;
+  s->trained = true;
+  s->train_loss = l;
+  if (loss) *loss = l;
+  return NB_OK;
+}
+
+// Sampled relative force error of the surrogate on the current state (no update): `samples`
+// bodies spread over the sorted order against the FP32 direct sum.  use_mlp: 0 off, 1 on.
+int nb_sur_error(nb_ctx *c, int32_t samples, int32_t use_mlp, double *err) {
+  if (!c || !err) return NB_E_INVALID;
+  if (!c->sur_on) return fail_msg(c, NB_E_STATE, "nb_sur_error: surrogate not enabled");
+  if (samples < 1 || samples > 65536) return fail_msg(c, NB_E_INVALID, "bad sample count");
+  Sur *s = c->sur;
+  NB_HIP(c, hipSetDevice(c->device));
+  NB_HIP(c, launch_zero_counters(c->ctr, 0, c->stream));
+  const bool mlp = use_mlp && s->trained;
+  int rc = sur_pipeline(c, false, mlp, true, mlp);
+  if (rc) return rc;
+  rc = sur_ensure_samples(c, samples);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sur_samples, dim3(nb256(samples)), dim3(256), 0, c->stream, s->sidx,
+                     samples, s->cstart, s->ncell);
+  hipLaunchKernelGGL(k_sur_exact, dim3((unsigned)samples), dim3(256), 0, c->stream, s->sp,
+                     s->cstart, s->ncell, s->sidx, (float)(c->p.eps * c->p.eps), s->sexact);
+  hipLaunchKernelGGL(k_sur_check, dim3(1), dim3(64), 0, c->stream, s->sidx, samples, s->perm,
+                     s->cstart, s->ncell, s->sexact, c->acc, (int)c->npad, 1e300, s->flag,
+                     s->serr);
+  NB_HIP(c, hipGetLastError());
+  NB_HIP(c, hipMemcpyAsync(err, s->serr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
+  return NB_OK;
+}
+
+int nb_sur_get_stats(nb_ctx *c, nb_sur_stats *out) {
+  if (!c || !out) return NB_E_INVALID;
+  memset(out, 0, sizeof *out);
+  if (!c->sur) return NB_OK;
+  Sur *s = c->sur;
+  NB_HIP(c, hipSetDevice(c->device));
+  DevCounters ctr;
+  double err = 0, pairs = 0;
+  int32_t nec = 0;
+  NB_HIP(c, hipMemcpyAsync(&ctr, c->ctr, sizeof ctr, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipMemcpyAsync(&err, s->serr, sizeof err, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipMemcpyAsync(&pairs, s->near_pairs, sizeof pairs, hipMemcpyDeviceToHost,
+                           c->stream));
+  NB_HIP(c, hipMemcpyAsync(&nec, s->necount, 4, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
+  out->steps = s->steps;
+  out->checks = s->checks;
+  out->fallbacks = (int64_t)ctr.sur_fallbacks;
+  out->last_error = err;
+  out->near_pairs_total = pairs;
+  out->nonempty_cells = nec;
+  out->train_rows = s->trows;
+  out->train_loss = s->train_loss;
+  out->r0 = s->r0;
+  out->trained = s->trained ? 1 : 0;
+  return NB_OK;
+}
+
+int nb_sur_save(nb_ctx *c, const char *path) {
+  if (!c || !path) return NB_E_INVALID;
+  Sur *s = c->sur;
+  if (!s || !s->trained) return fail_msg(c, NB_E_STATE, "nb_sur_save: no trained model");
+  int64_t cnt = 0;
+  nb_mlp_param_count(s->mlp, &cnt);
+  std::vector<float> prm((size_t)cnt), mu(kFeat), sd(kFeat);
+  int rc = nb_mlp_get_params(s->mlp, prm.data(), cnt);
+  if (rc) return fail_msg(c, rc, "nb_sur_save: reading parameters failed");
+  NB_HIP(c, hipMemcpy(mu.data(), s->mu, kFeat * 4, hipMemcpyDeviceToHost));
+  NB_HIP(c, hipMemcpy(sd.data(), s->sd, kFeat * 4, hipMemcpyDeviceToHost));
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail_msg(c, NB_E_INVALID, "nb_sur_save: cannot open file");
+  const char magic[8] = {'N', 'B', 'S', 'U', 'R', '0', '0', '1'};
+  const int32_t hdr[4] = {kFeat, 128, 3, 3};
+  bool ok = fwrite(magic, 1, 8, f) == 8 && fwrite(hdr, 4, 4, f) == 4 &&
+            fwrite(&cnt, 8, 1, f) == 1 && fwrite(mu.data(), 4, kFeat, f) == (size_t)kFeat &&
+            fwrite(sd.data(), 4, kFeat, f) == (size_t)kFeat &&
+            fwrite(&s->out_scale, 4, 1, f) == 1 &&
+            fwrite(prm.data(), 4, (size_t)cnt, f) == (size_t)cnt;
+  ok = (fclose(f) == 0) && ok;
+  return ok ? NB_OK : fail_msg(c, NB_E_INVALID, "nb_sur_save: write failed");
+}
+
+int nb_sur_load(nb_ctx *c, const char *path) {
+  if (!c || !path) return NB_E_INVALID;
+  Sur *s = c->sur;
+  if (!s) return fail_msg(c, NB_E_STATE, "nb_sur_load: enable the surrogate first");
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail_msg(c, NB_E_INVALID, "nb_sur_load: cannot open file");
+  char magic[8];
+  int32_t hdr[4];
+  int64_t cnt = 0;
+  std::vector<float> mu(kFeat), sd(kFeat);
+  float osc = 1.f;
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "NBSUR001", 8) == 0 &&
+            fread(hdr, 4, 4, f) == 4 && hdr[0] == kFeat && hdr[1] == 128 && hdr[2] == 3 &&
+            hdr[3] == 3 && fread(&cnt, 8, 1, f) == 1 && cnt > 0 && cnt < (1 << 24) &&
+            fread(mu.data(), 4, kFeat, f) == (size_t)kFeat &&
+            fread(sd.data(), 4, kFeat, f) == (size_t)kFeat && fread(&osc, 4, 1, f) == 1;
+  std::vector<float> prm;
+  if (ok) {
+    prm.resize((size_t)cnt);
+    ok = fread(prm.data(), 4, (size_t)cnt, f) == (size_t)cnt;
+  }
+  fclose(f);
+  if (!ok) return fail_msg(c, NB_E_INVALID, "nb_sur_load: not a surrogate model file");
+  if (!s->mlp) {
+    int rc = nb_mlp_create(c->device, kFeat, 128, 3, 3, s->p.seed, &s->mlp);
+    if (rc) return fail_msg(c, rc, "nb_sur_load: could not create the MLP");
+  }
+  int rc = nb_mlp_set_params(s->mlp, prm.data(), cnt);
+  if (rc) return fail_msg(c, rc, "nb_sur_load: parameter count mismatch");
+  NB_HIP(c, hipMemcpy(s->mu, mu.data(), kFeat * 4, hipMemcpyHostToDevice));
+  NB_HIP(c, hipMemcpy(s->sd, sd.data(), kFeat * 4, hipMemcpyHostToDevice));
+  s->out_scale = osc;
+  s->trained = true;
+  return NB_OK;
+}
+
+}  // extern "C"

@@ -187,6 +187,48 @@ class NBodyGPU:
         self._check(lib.nb_autotune(self._ctx, int(reps), cfg))
         return dict(zip(("ipl", "tile_j", "jsplit", "block", "variant"), list(cfg)))
 
+    # ---- learned far-field surrogate ----------------------------------------------------------
+    def sur_enable(self, grid=32, shell=2, r0=0.0, use_mlp=True, check_every=1,
+                   check_samples=256, tol=1e-3, seed=12345):
+        """Replace the O(N^2) force phase by the cell-grid multipole + MLP surrogate (FP32 path,
+        eps > 0, no collisions / overlap cutoff).  Steps whose sampled error exceeds tol are
+        redone by direct summation."""
+        p = L.NbSurParams()
+        lib.nb_sur_params_init(C.byref(p))
+        p.grid, p.shell, p.r0 = int(grid), int(shell), float(r0)
+        p.use_mlp, p.check_every = int(bool(use_mlp)), int(check_every)
+        p.check_samples, p.tol, p.seed = int(check_samples), float(tol), int(seed)
+        self._check(lib.nb_sur_enable(self._ctx, C.byref(p)))
+        return self
+
+    def sur_disable(self):
+        self._check(lib.nb_sur_disable(self._ctx))
+
+    def sur_collect(self, samples, seed=0):
+        """Append training rows from the loaded state, labelled by the FP32 direct sum."""
+        self._check(lib.nb_sur_collect(self._ctx, int(samples), int(seed)))
+
+    def sur_fit(self, steps=2000, batch=256, lr=1e-3):
+        loss = C.c_float()
+        self._check(lib.nb_sur_fit(self._ctx, int(steps), int(batch), float(lr), C.byref(loss)))
+        return loss.value
+
+    def sur_error(self, samples=1024, use_mlp=True):
+        err = C.c_double()
+        self._check(lib.nb_sur_error(self._ctx, int(samples), int(bool(use_mlp)), C.byref(err)))
+        return err.value
+
+    def sur_stats(self):
+        st = L.NbSurStats()
+        self._check(lib.nb_sur_get_stats(self._ctx, C.byref(st)))
+        return st.as_dict()
+
+    def sur_save(self, path):
+        self._check(lib.nb_sur_save(self._ctx, str(path).encode()))
+
+    def sur_load(self, path):
+        self._check(lib.nb_sur_load(self._ctx, str(path).encode()))
+
 
 class NBodyBatch:
     """Many independent simulations in one persistent, graph-captured launch (nb_batch_*)."""

@@ -38,7 +38,8 @@ def test_binding_covers_the_header():
 def test_struct_layouts_match_header(tmp_path):
     """sizeof/offsetof of every struct field as gcc lays out the header == the ctypes mirror."""
     import subprocess
-    structs = {"nb_params": L.NbParams, "nb_stats": L.NbStats, "nb_diag": L.NbDiag}
+    structs = {"nb_params": L.NbParams, "nb_stats": L.NbStats, "nb_diag": L.NbDiag,
+               "nb_sur_params": L.NbSurParams, "nb_sur_stats": L.NbSurStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nbody_gpu.h"',
              "int main(void){"]
     for cname, cls in structs.items():
@@ -88,3 +89,19 @@ def test_null_arguments_are_rejected_without_a_device():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(ImportError):
         L.load_library(str(tmp_path / "libnbgpu.so"))
+
+
+def test_surrogate_params_defaults_and_null_handling():
+    p = L.NbSurParams()
+    assert L.lib.nb_sur_params_init(C.byref(p)) == L.NB_OK
+    assert (p.grid, p.shell, p.r0, p.use_mlp) == (32, 2, 0.0, 1)
+    assert (p.check_every, p.check_samples, p.tol) == (1, 256, 1e-3)
+    assert L.lib.nb_sur_params_init(None) == L.NB_E_INVALID
+    assert L.lib.nb_sur_enable(None, C.byref(p)) == L.NB_E_INVALID
+    assert L.lib.nb_sur_disable(None) == L.NB_E_INVALID
+    assert L.lib.nb_sur_collect(None, 10, 0) == L.NB_E_INVALID
+    assert L.lib.nb_sur_fit(None, 1, 64, 1e-3, None) == L.NB_E_INVALID
+    st = L.NbSurStats()
+    assert L.lib.nb_sur_get_stats(None, C.byref(st)) == L.NB_E_INVALID
+    assert L.lib.nb_sur_save(None, b"x") == L.NB_E_INVALID
+    assert L.lib.nb_sur_load(None, b"x") == L.NB_E_INVALID

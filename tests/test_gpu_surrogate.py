@@ -1,0 +1,170 @@
+"""GPU tests of the learned far-field surrogate (nb_sur_*), against the CPU FP64 oracle.
+
+Bars:
+  all-near layout     every pair falls in adjacent cells, so the surrogate is an exact FP32 direct
+                      sum: global relative force error vs FP64 <= 1e-5 (the plain FP32 bar).
+  general layout      Plummer sphere, 32^3 grid, shell 2, no MLP: sampled global relative error
+                      vs FP64 <= 2e-3; a deeper shell lowers it.
+  fallback            with the tolerance below any reachable error every step is redone by the
+                      plain kernel, and the state is bit-identical to the plain path's.
+  learned residual    trained on direct-sum rows of seeded sims, the MLP lowers the held-out
+                      error by >= 20% (the test that shows what the network gains); a saved model
+                      reloads to bit-identical forces.
+"""
+import numpy as np
+import pytest
+
+from nbodygo_amd import NBodyGPU, NBodyError, gen
+from nbodygo_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+NT = 16
+EPS = 0.01
+
+
+def sim(**kw):
+    return NBodyGPU(G=1.0, eps=EPS, overlap_cutoff=False, collisions=False, **kw)
+
+
+def glob_err(got, exp):
+    return float(np.linalg.norm(got - exp) / np.linalg.norm(exp))
+
+
+def test_all_near_layout_is_exact_direct_sum(oracle):
+    # r0 far beyond the cluster: every body maps into the 2x2x2 central cells of a 4^3 grid.
+    b = gen.plummer(6000, seed=3)
+    st = oracle.State.from_bodies(b)
+    with sim() as g:
+        g.load(b)
+        g.sur_enable(grid=4, shell=2, r0=1.0e4, check_every=0)
+        f = g.forces()
+        stats = g.sur_stats()
+    ref = oracle.force_soft(st, 1.0, EPS, nthreads=NT)
+    assert glob_err(f, ref) <= 1e-5
+    assert stats["near_pairs_total"] == float(b.n) * b.n
+    assert stats["nonempty_cells"] <= 8
+
+
+def test_all_near_layout_skips_inactive_bodies(oracle):
+    b = gen.plummer(5000, seed=4)
+    b.flags[::7] = 0                       # removed bodies
+    b.flags[3::11] |= L.NB_F_FRAGMENTING   # fragmenting bodies exert and feel no force
+    st = oracle.State.from_bodies(b)
+    with sim() as g:
+        g.load(b)
+        g.sur_enable(grid=4, shell=2, r0=1.0e4, check_every=0)
+        f = g.forces()
+    act = (b.flags & L.NB_F_EXISTS).astype(bool) & ~(b.flags & L.NB_F_FRAGMENTING).astype(bool)
+    ref = oracle.force_soft(st, 1.0, EPS, nthreads=NT)
+    assert glob_err(f[act], ref[act]) <= 1e-5
+
+
+@pytest.mark.parametrize("n", [32768, 131072])
+def test_plummer_error_against_oracle(oracle, n):
+    b = gen.plummer(n, seed=5)
+    st = oracle.State.from_bodies(b)
+    idx = np.sort(np.random.default_rng(0).choice(n, 512, replace=False))
+    ref = oracle.force_soft(st, 1.0, EPS, idx=idx, nthreads=NT)
+    errs = {}
+    with sim() as g:
+        g.load(b)
+        for k in (2, 3):
+            g.sur_enable(shell=k, check_every=0, use_mlp=False)
+            errs[k] = glob_err(g.forces()[idx], ref)
+        internal = g.sur_error(1024, use_mlp=False)
+    assert errs[2] <= 2e-3, errs
+    assert errs[3] < errs[2], errs
+    # the library's own sampled check (FP32 direct sum over other samples) sees the same error
+    assert 0.5 < internal / errs[3] < 2.0
+
+
+def test_fallback_redo_is_bit_identical_to_plain_path():
+    b = gen.plummer(8192, seed=6)
+    with sim() as g:
+        g.load(b)
+        g.step(1e-3, nsteps=3)
+        plain = g.store()
+    with sim() as g:
+        g.load(b)
+        g.sur_enable(check_every=1, tol=1e-12)
+        g.step(1e-3, nsteps=3)
+        redo = g.store()
+        st = g.sur_stats()
+    assert st["checks"] == 3 and st["fallbacks"] == 3
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(getattr(redo, k), getattr(plain, k))
+
+
+def test_check_passes_under_loose_tolerance_and_is_deterministic():
+    b = gen.plummer(16384, seed=7)
+    out = []
+    for _ in range(2):
+        with sim() as g:
+            g.load(b)
+            g.sur_enable(check_every=2, tol=1e-2)
+            g.step(1e-3, nsteps=4)
+            st = g.sur_stats()
+            out.append(g.store())
+        assert st["checks"] == 2 and st["fallbacks"] == 0 and st["steps"] == 4
+        assert 0 < st["last_error"] < 1e-2
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(getattr(out[0], k), getattr(out[1], k))
+
+
+def test_unsupported_configurations_are_rejected():
+    b = gen.plummer(2048, seed=8)
+    for kw, code in ((dict(G=1.0, eps=EPS, collisions=True), L.NB_E_UNSUPPORTED),
+                     (dict(G=1.0, eps=EPS, overlap_cutoff=True, collisions=False),
+                      L.NB_E_UNSUPPORTED),
+                     (dict(G=1.0, eps=0.0, overlap_cutoff=False, collisions=False),
+                      L.NB_E_UNSUPPORTED),
+                     (dict(G=1.0, eps=EPS, overlap_cutoff=False, collisions=False,
+                           precision="fp64"), L.NB_E_UNSUPPORTED)):
+        with NBodyGPU(**kw) as g:
+            g.load(b)
+            with pytest.raises(NBodyError) as e:
+                g.sur_enable()
+            assert e.value.code == code
+    with sim() as g:
+        g.load(b)
+        for bad in (dict(grid=2), dict(grid=64), dict(shell=0), dict(tol=0.0), dict(r0=-1.0)):
+            with pytest.raises(NBodyError) as e:
+                g.sur_enable(**bad)
+            assert e.value.code == L.NB_E_INVALID
+        with pytest.raises(NBodyError):
+            g.sur_collect(100)  # not enabled yet
+        g.sur_enable()
+        with pytest.raises(NBodyError):
+            g.sur_fit(steps=10, batch=256)  # no rows collected
+
+
+def test_learned_residual_gain_on_held_out_sims(tmp_path):
+    n = 32768
+    with sim() as g:
+        for s in range(12):
+            g.load(gen.plummer(n, seed=1000 + s))
+            g.sur_enable(check_every=0)
+            g.sur_collect(8192, seed=s)
+        assert g.sur_stats()["train_rows"] == 12 * 8192
+        loss = g.sur_fit(steps=3000, batch=512, lr=1e-3)
+        assert 0 < loss < 0.6  # normalised residual variance left after training
+        path = tmp_path / "sur.bin"
+        g.sur_save(path)
+        gains, trained = [], []
+        for s in range(2):
+            g.load(gen.plummer(n, seed=5000 + s))
+            g.sur_enable(check_every=0)
+            e0 = g.sur_error(2048, use_mlp=False)
+            e1 = g.sur_error(2048, use_mlp=True)
+            gains.append(e1 / e0)
+            trained.append(e1)
+        f_trained = g.forces()
+    assert max(gains) <= 0.8, gains
+    assert max(trained) <= 1e-3, trained
+    with sim() as h:
+        h.load(gen.plummer(n, seed=5001))
+        h.sur_enable(check_every=0)
+        h.sur_load(path)
+        assert h.sur_error(2048, use_mlp=True) == trained[1]
+        np.testing.assert_array_equal(h.forces(), f_trained)
