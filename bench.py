@@ -10,6 +10,11 @@ nbody_computations counter), steps/s, the force kernel's FP32 roofline fraction 
 HIP events on the library's stream, the force error against an FP64 direct sum, and the
 reference's CPU algorithm (worker pool of threads over contiguous target slices) timed on a
 bounded sample of the same workload on this host.
+
+The "surrogate" object compares, on a Plummer sphere of N = 262,144 bodies, the plain direct-sum
+step with the learned far-field surrogate step (32^3 stretched grid, exact near field, MLP
+residual trained on direct-sum rows of 200 seeded sims of the same size; the sampled error check
+and its fallback run inside the timed steps), with the surrogate's force error against FP64.
 """
 import argparse
 import json
@@ -81,6 +86,9 @@ def parse():
     ap.add_argument("--shape", default="",
                     help="pin the FP32 force kernel shape: ipl,tile_j,jsplit,block,variant")
     ap.add_argument("--no-elastic", action="store_true", help="skip the elastic-collision run")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate comparison")
+    ap.add_argument("--sur-sims", type=int, default=200, help="training sims for the surrogate")
+    ap.add_argument("--sur-steps", type=int, default=50, help="timed surrogate steps")
     return ap.parse_args()
 
 
@@ -115,6 +123,53 @@ def timed_run(g, b, steps, warmup, dt=1e-3):
     t1 = time.perf_counter()
     g.set_timing(False)
     return s, t1 - t0
+
+
+def surrogate_leg(a, pinned):
+    """Plain vs surrogate at N = 262,144 (Plummer, eps = 0.01, dt = 1e-3, collisions off)."""
+    from nbodygo_amd import NBodyGPU, gen
+    n = 262144
+    b = gen.plummer(n, seed=2024)
+    g = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=False, **pinned)
+    g.load(b)
+    shape = g.config() if (a.no_tune or pinned) else g.autotune(reps=2)
+    ps, pel = timed_run(g, b, 10, 3)
+    plain_ms = pel / 10 * 1e3
+    # training rows: 2048 bodies from each of the seeded sims, labelled by the FP32 direct sum
+    t0 = time.perf_counter()
+    for k in range(a.sur_sims):
+        g.load(gen.plummer(n, seed=100000 + k))
+        g.sur_enable(check_every=0)
+        g.sur_collect(2048, seed=k)
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = g.sur_fit(steps=4000, batch=512, lr=1e-3)
+    t_fit = time.perf_counter() - t0
+    rows = g.sur_stats()["train_rows"]
+    # held-out state: error without / with the learned residual, against FP64
+    g.load(b)
+    g.sur_enable(check_every=1, tol=1e-3)
+    err_nomlp = g.sur_error(4096, use_mlp=False)
+    err_mlp = force_error(g, b)
+    s0 = g.sur_stats()
+    ss, sel = timed_run(g, b, a.sur_steps, 3)
+    s1 = g.sur_stats()
+    sur_ms = sel / a.sur_steps * 1e3
+    g.close()
+    return {
+        "n": n, "workload": "Plummer sphere N=262144 (a=1, M=1, G=1), eps=0.01, dt=1e-3, "
+                            "collisions off; 32^3 stretched grid, shell 2, MLP 3x128 GELU",
+        "ms_per_step": sur_ms, "steps_per_s": 1e3 / sur_ms,
+        "equiv_G_interactions_per_s": ss["interactions"] / sel / 1e9,
+        "plain_ms_per_step": plain_ms, "plain_steps_per_s": 1e3 / plain_ms,
+        "plain_G_interactions_per_s": ps["interactions"] / pel / 1e9,
+        "plain_kernel_shape": shape, "speedup_vs_plain": plain_ms / sur_ms,
+        "force_error": err_mlp, "error_without_mlp": err_nomlp,
+        "checks": s1["checks"] - s0["checks"], "fallbacks": s1["fallbacks"] - s0["fallbacks"],
+        "tolerance": 1e-3, "timed_steps": a.sur_steps,
+        "training": {"sims": a.sur_sims, "rows": rows, "adam_steps": 4000, "batch": 512,
+                     "loss": loss, "collect_s": t_collect, "fit_s": t_fit},
+    }
 
 
 def cpu_baseline(b, seconds):
@@ -195,9 +250,11 @@ def main():
                           "ms_per_step": ele / a.steps * 1e3, "pairs_total": se["pairs_total"],
                           "fixup_tiles": se["fixup_tiles"], "radius": 1e-4}
         ge.close()
+    g.close()
+    if a.workload == "plummer65k" and not a.no_surrogate:
+        out["surrogate"] = surrogate_leg(a, pinned)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(b, a.cpu_seconds)
-    g.close()
     print(json.dumps(out))
 
 
