@@ -221,6 +221,9 @@ int nb_mlp_set_params(nb_mlp *m, const float *params, int64_t count);
 int nb_mlp_get_params(nb_mlp *m, float *params, int64_t count);
 /* y [rows x d_out] = MLP(x [rows x d_in]) */
 int nb_mlp_forward(nb_mlp *m, const float *x, int64_t rows, float *y);
+/* The same outputs, bit for bit, from the fused one-kernel inference path the surrogate uses
+ * (padded d_in and hidden <= 128, hidden a multiple of 64, d_out <= 16). */
+int nb_mlp_infer(nb_mlp *m, const float *x, int64_t rows, float *y);
 /* Gradient of the mean squared error over rows (a multiple of 64) w.r.t. the parameters. */
 int nb_mlp_grad(nb_mlp *m, const float *x, const float *t, int64_t rows, float *grad,
                 float *loss);

@@ -161,6 +161,7 @@ SIGNATURES = [
     ("nb_mlp_set_params", C.c_int, [_ctx, _fp, C.c_int64]),
     ("nb_mlp_get_params", C.c_int, [_ctx, _fp, C.c_int64]),
     ("nb_mlp_forward", C.c_int, [_ctx, _fp, C.c_int64, _fp]),
+    ("nb_mlp_infer", C.c_int, [_ctx, _fp, C.c_int64, _fp]),
     ("nb_mlp_grad", C.c_int, [_ctx, _fp, _fp, C.c_int64, _fp, _fp]),
     ("nb_mlp_train", C.c_int,
      [_ctx, _fp, _fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, _fp]),

@@ -34,15 +34,22 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// tanh(u) = 1 - 2 / (e^(2u) + 1), with the hardware base-2 exponential and reciprocal: an
+// absolute error of a few 1e-8 (what GELU's 1 + tanh needs), saturating to +-1 for large |u|.
+__device__ __forceinline__ float tanh_fast(float u) {
+  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * u);  // 2 log2(e) u
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
+
 // GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
 __device__ __forceinline__ float gelu(float x) {
   const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-  return 0.5f * x * (1.f + tanhf(k0 * (x + k1 * x * x * x)));
+  return 0.5f * x * (1.f + tanh_fast(k0 * (x + k1 * x * x * x)));
 }
 __device__ __forceinline__ float gelu_grad(float x) {
   const float k0 = 0.7978845608028654f, k1 = 0.044715f;
   const float u = k0 * (x + k1 * x * x * x);
-  const float t = tanhf(u);
+  const float t = tanh_fast(u);
   return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k0 * (1.f + 3.f * k1 * x * x);
 }
 
@@ -193,6 +200,113 @@ __global__ void k_adam(float *w, const float *g, float *m1, float *m2, uint16_t 
   if (wb) wb[i] = f2bf(wi);
 }
 
+// [K][N] -> [N][K] (BF16), so that an MFMA B fragment (four consecutive k of one column) is one
+// 8-byte load.
+__global__ void k_transpose_bf16(const uint16_t *src, int K, int N, uint16_t *dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K * N) return;
+  const int k = i / N, n = i % N;
+  dst[(size_t)n * K + k] = src[i];
+}
+
+// Fused inference: every layer of the MLP for 64 rows per workgroup, activations kept in LDS as
+// BF16 (exactly what the layer-by-layer path stores between its GEMMs), weights read from the
+// transposed BF16 copies.  The MFMA sequence per output element (K in steps of 16 from 0) and
+// the epilogue (bias, GELU, BF16 rounding) are those of k_gemm_bf16, so the outputs equal the
+// layer-by-layer forward bit for bit.  Limits: padded input width and hidden width <= 128,
+// hidden width a multiple of 64, d_out <= 16.
+constexpr int kInferW = 128;
+constexpr int kInferMaxLayers = 6;
+struct InferArgs {
+  const float *x;
+  int rows, din, dinp, hid, layers, dout;
+  const uint16_t *wt[kInferMaxLayers + 1];  // [out][in]
+  const float *b[kInferMaxLayers + 1];
+  float *y;  // [rows x dout]
+};
+
+// One hidden layer for the workgroup's 64 rows: Hout = BF16(GELU(Hin * W + b)).  Wave w owns
+// columns [w HID/4, (w+1) HID/4); all B fragments of the layer are fetched up front.
+template <int K, int HID>
+__device__ __forceinline__ void infer_layer(const uint16_t (*Hin)[kInferW + 8],
+                                            uint16_t (*Hout)[kInferW + 8], const uint16_t *wt,
+                                            const float *bias, int w, int lr, int lq) {
+  constexpr int NBW = HID / 64, KS = K / 16;
+  bf16x4 fb[KS][NBW];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+    for (int b = 0; b < NBW; b++)
+      fb[ks][b] = *(const bf16x4 *)(wt + (size_t)(w * (HID / 4) + b * 16 + lr) * K + ks * 16 + lq);
+  f32x4 acc[4][NBW];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int b = 0; b < NBW; b++) acc[m][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    bf16x4 fa[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) fa[m] = *(const bf16x4 *)&Hin[m * 16 + lr][ks * 16 + lq];
+#pragma unroll
+    for (int b = 0; b < NBW; b++)
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+        acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[m], fb[ks][b], acc[m][b], 0, 0, 0);
+  }
+#pragma unroll
+  for (int b = 0; b < NBW; b++) {
+    const int col = w * (HID / 4) + b * 16 + lr;
+    const float bc = bias[col];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) Hout[m * 16 + lq + v][col] = f2bf(gelu(acc[m][b][v] + bc));
+  }
+}
+
+template <int DINP, int HID>
+__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t Hs[2][64][kInferW + 8];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r0 = blockIdx.x * 64;
+  for (int i = tid; i < 64 * DINP; i += 256) {
+    const int r = i / DINP, c = i % DINP, row = r0 + r;
+    Hs[0][r][c] = (row < a.rows && c < a.din) ? f2bf(a.x[(size_t)row * a.din + c]) : 0;
+  }
+  __syncthreads();
+  const int lr = lane & 15, lq = 4 * (lane >> 4);
+  infer_layer<DINP, HID>(Hs[0], Hs[1], a.wt[0], a.b[0], w, lr, lq);
+  __syncthreads();
+  int cur = 1;
+  for (int l = 1; l < a.layers; l++) {
+    infer_layer<HID, HID>(Hs[cur], Hs[cur ^ 1], a.wt[l], a.b[l], w, lr, lq);
+    __syncthreads();
+    cur ^= 1;
+  }
+  // output layer, columns 0..15; wave w takes rows 16w .. 16w+15
+  constexpr int KS = HID / 16;
+  const uint16_t *wt = a.wt[a.layers];
+  bf16x4 fb[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) fb[ks] = *(const bf16x4 *)(wt + (size_t)lr * HID + ks * 16 + lq);
+  f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    const bf16x4 fa = *(const bf16x4 *)&Hs[cur][w * 16 + lr][ks * 16 + lq];
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb[ks], acc, 0, 0, 0);
+  }
+  if (lr < a.dout) {
+    const float bc = a.b[a.layers][lr];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = r0 + w * 16 + lq + v;
+      if (row < a.rows) a.y[(size_t)row * a.dout + lr] = acc[v] + bc;
+    }
+  }
+}
+
 }  // namespace nb
 
 // ---------------------------------------------------------------------------------------------
@@ -221,6 +335,9 @@ struct nb_mlp {
   float *xin = nullptr;     // FP32 input staging [rows x din]
   int64_t adam_t = 0;
   int64_t splitbuf_len = 0;
+  // transposed BF16 weights for the fused inference kernel, rebuilt when wver moves
+  std::vector<uint16_t *> WbT;
+  int64_t wver = 0, wt_ver = -1;
 };
 
 namespace {
@@ -382,6 +499,7 @@ int backward(nb_mlp *m, int64_t rows) {
 
 int adam_step(nb_mlp *m, float lr) {
   m->adam_t++;
+  m->wver++;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - powf(b1, (float)m->adam_t), c2 = 1.f - powf(b2, (float)m->adam_t);
   for (int l = 0; l <= m->layers; l++) {
@@ -448,6 +566,55 @@ int mlp_train_step_device(nb_mlp *m, const float *x_dev, const float *t_dev, int
   m->stream = keep;
   return rc;
 }
+// Fused inference on device-resident FP32 rows [rows x din] into y [rows x dout].
+int mlp_infer_device(nb_mlp *m, const float *x_dev, int64_t rows, float *y_dev,
+                     hipStream_t st) {
+  if (m->dinp > kInferW || m->hid > kInferW || m->hid % 64 || m->dinp % 64 || m->dout > 16 ||
+      m->layers > kInferMaxLayers || m->layers < 1) {
+    m->err = "fused inference: shape beyond its limits";
+    return NB_E_UNSUPPORTED;
+  }
+  if (rows <= 0) return NB_OK;
+  if (m->wt_ver != m->wver) {
+    if (m->WbT.empty())
+      for (int l = 0; l <= m->layers; l++) {
+        uint16_t *q = nullptr;
+        NB_MHIP(m, hipMalloc((void **)&q, (size_t)m->in_dim[l] * m->out_dim[l] * 2));
+        m->WbT.push_back(q);
+      }
+    for (int l = 0; l <= m->layers; l++) {
+      const int K = m->in_dim[l], N = m->out_dim[l];
+      hipLaunchKernelGGL(k_transpose_bf16, dim3(nblk((int64_t)K * N)), dim3(256), 0, st,
+                         m->Wb[l], K, N, m->WbT[l]);
+    }
+    NB_MHIP(m, hipGetLastError());
+    m->wt_ver = m->wver;
+  }
+  InferArgs a;
+  a.x = x_dev;
+  a.rows = (int)rows;
+  a.din = m->din;
+  a.dinp = m->dinp;
+  a.hid = m->hid;
+  a.layers = m->layers;
+  a.dout = m->dout;
+  for (int l = 0; l <= kInferMaxLayers; l++) {
+    a.wt[l] = l <= m->layers ? m->WbT[l] : nullptr;
+    a.b[l] = l <= m->layers ? m->bvec[l] : nullptr;
+  }
+  a.y = y_dev;
+  const dim3 grid((unsigned)((rows + 63) / 64));
+  const int key = (m->dinp == 64 ? 0 : 1) + (m->hid == 64 ? 0 : 2);
+  switch (key) {
+    case 0: hipLaunchKernelGGL((k_mlp_infer<64, 64>), grid, dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_mlp_infer<128, 64>), grid, dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_mlp_infer<64, 128>), grid, dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_mlp_infer<128, 128>), grid, dim3(256), 0, st, a); break;
+  }
+  NB_MHIP(m, hipGetLastError());
+  return NB_OK;
+}
+
 int mlp_dims(const nb_mlp *m, int *din, int *dout) {
   *din = m->din;
   *dout = m->dout;
@@ -546,12 +713,34 @@ void nb_mlp_destroy(nb_mlp *m) {
   for (auto *v : {&m->W, &m->Wg, &m->Wm1, &m->Wm2, &m->bvec, &m->bg, &m->bm1, &m->bm2})
     for (float *q : *v) (void)hipFree(q);
   for (uint16_t *q : m->Wb) (void)hipFree(q);
+  for (uint16_t *q : m->WbT) (void)hipFree(q);
   if (m->splitbuf) (void)hipFree(m->splitbuf);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
 
 const char *nb_mlp_last_error(const nb_mlp *m) { return m ? m->err.c_str() : "NULL mlp"; }
+
+int nb_mlp_infer(nb_mlp *m, const float *x, int64_t rows, float *y) {
+  if (!m || (rows > 0 && (!x || !y)) || rows < 0) return NB_E_INVALID;
+  if (rows == 0) return NB_OK;
+  NB_MHIP(m, hipSetDevice(m->device));
+  float *dx = nullptr, *dy = nullptr;
+  NB_MHIP(m, hipMalloc((void **)&dx, (size_t)rows * m->din * 4));
+  hipError_t e = hipMalloc((void **)&dy, (size_t)rows * m->dout * 4);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dx, x, (size_t)rows * m->din * 4, hipMemcpyHostToDevice, m->stream);
+  int rc = e == hipSuccess ? mlp_infer_device(m, dx, rows, dy, m->stream) : NB_E_HIP;
+  if (!rc) {
+    e = hipMemcpyAsync(y, dy, (size_t)rows * m->dout * 4, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) rc = NB_E_HIP;
+  }
+  if (dx) (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  if (rc == NB_E_HIP && m->err.empty()) m->err = "nb_mlp_infer: HIP error";
+  return rc;
+}
 
 // Parameters in the unpadded layout: for each layer W [in_real x out_real] row-major, then b.
 int nb_mlp_param_count(const nb_mlp *m, int64_t *count) {
@@ -606,6 +795,7 @@ int nb_mlp_set_params(nb_mlp *m, const float *p, int64_t count) {
   if (!m || !p) return NB_E_INVALID;
   int rc = xfer_params(m, const_cast<float *>(p), count, true, false);
   m->adam_t = 0;
+  m->wver++;
   return rc;
 }
 

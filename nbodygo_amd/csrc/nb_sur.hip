@@ -33,8 +33,8 @@
 
 namespace nb {
 
-int mlp_forward_device(nb_mlp *m, const float *x_dev, int64_t rows, const float **y_dev,
-                       int *ldy, hipStream_t st);
+int mlp_infer_device(nb_mlp *m, const float *x_dev, int64_t rows, float *y_dev,
+                     hipStream_t st);
 int mlp_train_step_device(nb_mlp *m, const float *x_dev, const float *t_dev, int64_t rows,
                           float lr, hipStream_t st);
 
@@ -59,7 +59,7 @@ struct Sur {
   int32_t *cstart = nullptr, *ccount = nullptr, *nelist = nullptr, *necount = nullptr;
   float4 *sp = nullptr;
   float *crec = nullptr, *crec_ne = nullptr, *fexp = nullptr, *part = nullptr, *feat = nullptr;
-  int2 *items = nullptr;
+  int2 *items = nullptr, *btot = nullptr;
   int32_t *nitems = nullptr;
   int64_t items_cap = 0;
   float4 *base = nullptr;
@@ -72,7 +72,9 @@ struct Sur {
   int64_t trows = 0, tcap = 0;
   int32_t *sidx = nullptr;
   int S = 0;
-  double *sexact = nullptr, *serr = nullptr;
+  double *sexact = nullptr, *serr = nullptr, *spart = nullptr;
+  int64_t spart_len = 0;
+  float *ymlp = nullptr;  // MLP output [cap x 3]
   uint32_t *flag = nullptr;
   int64_t steps = 0, fallbacks = 0, checks = 0;
   double last_err = 0;
@@ -157,24 +159,20 @@ __global__ void k_sur_keys(const double *x, const double *y, const double *z,
 }
 
 // cstart[c] = first sorted position whose key is >= c, for c = 0..ncell (inactive bodies carry
-// key ncell and sort last, so cstart[ncell] = active bodies).  Binary search, no atomics.
-__global__ void k_sur_bounds(const uint32_t *keys2, int n, int ncell, int32_t *cstart) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > ncell) return;
+// key ncell and sort last, so cstart[ncell] = active bodies).  Binary search, no atomics.  Each
+// 1024-cell block also totals its nonempty cells and near-field work items (64-body chunks).
+__device__ __forceinline__ int lower_bound_key(const uint32_t *keys, int n, int c) {
   int lo = 0, hi = n;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if ((int)keys2[mid] < c)
+    if ((int)keys[mid] < c)
       lo = mid + 1;
     else
       hi = mid;
   }
-  cstart[c] = lo;
+  return lo;
 }
 
-// Per-cell counts, the list of nonempty cells and the near-field work items (cell, 64-body chunk),
-// by one workgroup of 1024 threads walking the cells in rounds of 1024 (one cell per thread):
-// wave prefix sums, then the 16 wave totals, then a running carry -- a fixed-order scan.
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -184,51 +182,82 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
   return v;
 }
 
-__global__ __launch_bounds__(1024) void k_sur_scan(const int32_t *cstart, int ncell,
-                                                   int32_t *ccount, int32_t *nelist,
-                                                   int32_t *necount, int2 *items,
-                                                   int32_t *nitems) {
-  __shared__ int32_t wa[16], wb[16], carry[2];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t == 0) carry[0] = carry[1] = 0;
-  __syncthreads();
-  for (int base = 0; base < ncell; base += 1024) {
-    const int c = base + t;
-    const int cnt = c < ncell ? cstart[c + 1] - cstart[c] : 0;
-    const int a = cnt > 0, b = (cnt + 63) >> 6;
-    const int ia = wave_incl_scan(a, lane), ib = wave_incl_scan(b, lane);
-    if (lane == 63) {
-      wa[w] = ia;
-      wb[w] = ib;
-    }
-    __syncthreads();
-    if (w == 0) {
-      const int va = wave_incl_scan(lane < 16 ? wa[lane] : 0, lane);
-      const int vb = wave_incl_scan(lane < 16 ? wb[lane] : 0, lane);
-      if (lane < 16) {
-        wa[lane] = va;
-        wb[lane] = vb;
-      }
-    }
-    __syncthreads();
-    const int offa = carry[0] + (w > 0 ? wa[w - 1] : 0) + ia - a;
-    const int offb = carry[1] + (w > 0 ? wb[w - 1] : 0) + ib - b;
+__global__ __launch_bounds__(1024) void k_sur_bounds(const uint32_t *keys2, int n, int ncell,
+                                                     int32_t *cstart, int2 *btot) {
+  __shared__ int32_t wa[16], wb[16];
+  const int c = blockIdx.x * 1024 + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int a = 0, b = 0;
+  if (c <= ncell) {
+    const int lo = lower_bound_key(keys2, n, c);
+    cstart[c] = lo;
     if (c < ncell) {
-      ccount[c] = cnt;
-      if (a) nelist[offa] = c;
-      for (int j = 0; j < b; j++) items[offb + j] = make_int2(c, j);
+      const int cnt = lower_bound_key(keys2, n, c + 1) - lo;
+      a = cnt > 0;
+      b = (cnt + 63) >> 6;
     }
-    __syncthreads();
-    if (t == 0) {
-      carry[0] += wa[15];
-      carry[1] += wb[15];
+  }
+  a = wave_incl_scan(a, lane);
+  b = wave_incl_scan(b, lane);
+  if (lane == 63) {
+    wa[w] = a;
+    wb[w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sa = 0, sb = 0;
+    for (int q = 0; q < 16; q++) {
+      sa += wa[q];
+      sb += wb[q];
     }
-    __syncthreads();
+    btot[blockIdx.x] = make_int2(sa, sb);
   }
-  if (t == 0) {
-    *necount = carry[0];
-    *nitems = carry[1];
+}
+
+// Per-cell counts, the list of nonempty cells and the near-field work items (cell, 64-body
+// chunk): 1024 cells per workgroup, block offsets from the block totals, in-block wave prefix
+// sums -- integer scans, so the order is fixed.
+__global__ __launch_bounds__(1024) void k_sur_scan(const int32_t *cstart, int ncell,
+                                                   const int2 *btot, int32_t *ccount,
+                                                   int32_t *nelist, int32_t *necount, int2 *items,
+                                                   int32_t *nitems) {
+  __shared__ int32_t wa[16], wb[16], base[2];
+  const int c = blockIdx.x * 1024 + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) {
+    int sa = 0, sb = 0;
+    for (int q = 0; q < (int)blockIdx.x; q++) {
+      sa += btot[q].x;
+      sb += btot[q].y;
+    }
+    base[0] = sa;
+    base[1] = sb;
+    if (blockIdx.x == gridDim.x - 1) {
+      *necount = sa + btot[blockIdx.x].x;
+      *nitems = sb + btot[blockIdx.x].y;
+    }
   }
+  const int cnt = c < ncell ? cstart[c + 1] - cstart[c] : 0;
+  const int a = cnt > 0, b = (cnt + 63) >> 6;
+  const int ia = wave_incl_scan(a, lane), ib = wave_incl_scan(b, lane);
+  if (lane == 63) {
+    wa[w] = ia;
+    wb[w] = ib;
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int va = wave_incl_scan(lane < 16 ? wa[lane] : 0, lane);
+    const int vb = wave_incl_scan(lane < 16 ? wb[lane] : 0, lane);
+    if (lane < 16) {
+      wa[lane] = va;
+      wb[lane] = vb;
+    }
+  }
+  __syncthreads();
+  if (c >= ncell) return;
+  const int y = base[0] + (w > 0 ? wa[w - 1] : 0) + ia - a;
+  const int q = base[1] + (w > 0 ? wb[w - 1] : 0) + ib - b;
+  ccount[c] = cnt;
+  if (a) nelist[y] = c;
+  for (int j = 0; j < b; j++) items[q + j] = make_int2(c, j);
 }
 
 __global__ void k_sur_gather(const uint32_t *perm, const uint32_t *keys2, int n, int ncell,
@@ -252,6 +281,14 @@ __device__ __forceinline__ double wmax(double v) {
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
   return v;
+}
+
+// Orders this wave's writes of its private LDS tile before its later reads (and the reads before
+// the next overwrite); the waitcnt of a workgroup-scope fence covers the LDS queue.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // Cell moments about the cell's centre of mass (FP64 accumulation), one wave per nonempty cell:
@@ -324,20 +361,24 @@ __global__ __launch_bounds__(64) void k_sur_moments(const float4 *sp, const int3
 // quadrupole field Q r / rs^5 - 5/2 (r.Q.r) r / rs^7; r = X_target - X_source.
 //
 // Work unit = (64 target cells, one lane each) x (kM2LSlice source cells); the four waves of a
-// workgroup take a quarter of the slice each, streaming source records through the scalar cache,
-// and are summed in LDS in wave order.  Partials per slice go to `part`, summed by k_sur_m2l_sum.
+// workgroup take a quarter of the slice each, staging 64 source records at a time in a private
+// LDS tile (one record per lane) and reading them back as wave-uniform broadcasts, so every VALU
+// operand is a VGPR; the waves are summed in LDS in wave order.  Partials per slice go to
+// `part`, summed by k_sur_m2l_sum.
 constexpr int kM2LSlice = 1024;
 
 __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int32_t *necount,
                                                  int K, float eps2_s, int ncell, float *part) {
   __shared__ float red[3][kFarRec - 1][64];
+  __shared__ float4 recs[4][64][4];
   const int nec = *necount;
   const int ntg = (nec + 63) / 64, nsl = (nec + kM2LSlice - 1) / kM2LSlice;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const float eps2 = to_vgpr(eps2_s);
   const float Kf = (float)K + 0.5f;
-  cf4p src4 = (cf4p)crec_ne;
+  const float4 *src4 = (const float4 *)crec_ne;
+  float4 (*rl)[4] = recs[wave];
   for (int u = blockIdx.x; u < ntg * nsl; u += gridDim.x) {
     const int tg = u % ntg, sl = u / ntg;
     const int t = tg * 64 + lane;
@@ -349,54 +390,62 @@ __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int
     for (int q = 0; q < kFarRec - 1; q++) o[q] = 0.f;
     const int per = kM2LSlice / 4;
     const int s0 = sl * kM2LSlice + wave * per, s1 = min(s0 + per, nec);
+    for (int sb = s0; sb < s1; sb += 64) {
+      const int cnt = min(64, s1 - sb);
+      // padding records are all zero: no mass, no quadrupole, no contribution
+      for (int q = 0; q < 4; q++)
+        rl[lane][q] = lane < cnt ? src4[(size_t)(sb + lane) * 4 + q]
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      wave_lds_sync();
 #pragma unroll 2
-    for (int s = s0; s < s1; s++) {
-      const f4v r0 = src4[(size_t)s * 4 + 0], r1 = src4[(size_t)s * 4 + 1],
-                r2v = src4[(size_t)s * 4 + 2], r3 = src4[(size_t)s * 4 + 3];
-      // r0 = COM, M; r1 = Qxx Qyy Qzz Qxy; r2 = Qxz Qyz radius ix; r3 = iy iz n 0
-      const float cheb = fmaxf(fabsf(r2v.w - ix), fmaxf(fabsf(r3.x - iy), fabsf(r3.y - iz)));
-      const float rx = X - r0.x, ry = Y - r0.y, rz = Z - r0.z;
-      float rr = __builtin_fmaf(rx, rx, eps2);
-      rr = __builtin_fmaf(ry, ry, rr);
-      rr = __builtin_fmaf(rz, rz, rr);
-      float inv = __builtin_amdgcn_rsqf(rr);
-      inv = cheb > Kf ? inv : 0.f;
-      const float inv2 = inv * inv, M = r0.w;
-      const float m3 = M * inv2 * inv, m5 = m3 * inv2, m7 = m5 * inv2;
-      o[0] -= m3 * rx;
-      o[1] -= m3 * ry;
-      o[2] -= m3 * rz;
-      // quadrupole field
-      const float qx = r1.x * rx + r1.w * ry + r2v.x * rz;
-      const float qy = r1.w * rx + r1.y * ry + r2v.y * rz;
-      const float qz = r2v.x * rx + r2v.y * ry + r1.z * rz;
-      const float rqr = qx * rx + qy * ry + qz * rz;
-      const float i5 = inv2 * inv2 * inv, i7 = i5 * inv2;
-      const float c7 = 2.5f * rqr * i7;
-      o[0] += qx * i5 - c7 * rx;
-      o[1] += qy * i5 - c7 * ry;
-      o[2] += qz * i5 - c7 * rz;
-      // Jacobian of the monopole field (xx yy zz xy xz yz)
-      const float t3 = 3.f * m5;
-      o[3] += t3 * rx * rx - m3;
-      o[4] += t3 * ry * ry - m3;
-      o[5] += t3 * rz * rz - m3;
-      o[6] += t3 * rx * ry;
-      o[7] += t3 * rx * rz;
-      o[8] += t3 * ry * rz;
-      // Hessian of the monopole field (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz)
-      const float f5 = 15.f * m7;
-      const float ax5 = t3 * rx, ay5 = t3 * ry, az5 = t3 * rz;
-      o[9] += 3.f * ax5 - f5 * rx * rx * rx;
-      o[10] += ay5 - f5 * rx * rx * ry;
-      o[11] += az5 - f5 * rx * rx * rz;
-      o[12] += ax5 - f5 * rx * ry * ry;
-      o[13] += -f5 * rx * ry * rz;
-      o[14] += ax5 - f5 * rx * rz * rz;
-      o[15] += 3.f * ay5 - f5 * ry * ry * ry;
-      o[16] += az5 - f5 * ry * ry * rz;
-      o[17] += ay5 - f5 * ry * rz * rz;
-      o[18] += 3.f * az5 - f5 * rz * rz * rz;
+      for (int j = 0; j < cnt; j++) {
+        const float4 r0 = rl[j][0], r1 = rl[j][1], r2v = rl[j][2], r3 = rl[j][3];
+        // r0 = COM, M; r1 = Qxx Qyy Qzz Qxy; r2 = Qxz Qyz radius ix; r3 = iy iz n 0
+        const float cheb = fmaxf(fabsf(r2v.w - ix), fmaxf(fabsf(r3.x - iy), fabsf(r3.y - iz)));
+        const float rx = X - r0.x, ry = Y - r0.y, rz = Z - r0.z;
+        float rr = __builtin_fmaf(rx, rx, eps2);
+        rr = __builtin_fmaf(ry, ry, rr);
+        rr = __builtin_fmaf(rz, rz, rr);
+        float inv = __builtin_amdgcn_rsqf(rr);
+        inv = cheb > Kf ? inv : 0.f;
+        const float inv2 = inv * inv, M = r0.w;
+        const float m3 = M * inv2 * inv, m5 = m3 * inv2, m7 = m5 * inv2;
+        o[0] -= m3 * rx;
+        o[1] -= m3 * ry;
+        o[2] -= m3 * rz;
+        // quadrupole field
+        const float qx = r1.x * rx + r1.w * ry + r2v.x * rz;
+        const float qy = r1.w * rx + r1.y * ry + r2v.y * rz;
+        const float qz = r2v.x * rx + r2v.y * ry + r1.z * rz;
+        const float rqr = qx * rx + qy * ry + qz * rz;
+        const float i5 = inv2 * inv2 * inv, i7 = i5 * inv2;
+        const float c7 = 2.5f * rqr * i7;
+        o[0] += qx * i5 - c7 * rx;
+        o[1] += qy * i5 - c7 * ry;
+        o[2] += qz * i5 - c7 * rz;
+        // Jacobian of the monopole field (xx yy zz xy xz yz)
+        const float t3 = 3.f * m5;
+        o[3] += t3 * rx * rx - m3;
+        o[4] += t3 * ry * ry - m3;
+        o[5] += t3 * rz * rz - m3;
+        o[6] += t3 * rx * ry;
+        o[7] += t3 * rx * rz;
+        o[8] += t3 * ry * rz;
+        // Hessian of the monopole field (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz)
+        const float f5 = 15.f * m7;
+        const float ax5 = t3 * rx, ay5 = t3 * ry, az5 = t3 * rz;
+        o[9] += 3.f * ax5 - f5 * rx * rx * rx;
+        o[10] += ay5 - f5 * rx * rx * ry;
+        o[11] += az5 - f5 * rx * rx * rz;
+        o[12] += ax5 - f5 * rx * ry * ry;
+        o[13] += -f5 * rx * ry * rz;
+        o[14] += ax5 - f5 * rx * rz * rz;
+        o[15] += 3.f * ay5 - f5 * ry * ry * ry;
+        o[16] += az5 - f5 * ry * ry * rz;
+        o[17] += ay5 - f5 * ry * rz * rz;
+        o[18] += 3.f * az5 - f5 * rz * rz * rz;
+      }
+      wave_lds_sync();
     }
     if (wave > 0)
       for (int q = 0; q < kFarRec - 1; q++) red[wave - 1][q][lane] = o[q];
@@ -457,6 +506,11 @@ __device__ __forceinline__ void near_pair(f4v b, float4 me, float eps2, float &p
   pz = __builtin_fmaf(dz, s, pz);
 }
 
+__device__ __forceinline__ void near_pair(float4 b, float4 me, float eps2, float &px, float &py,
+                                          float &pz) {
+  near_pair(f4v{b.x, b.y, b.z, b.w}, me, eps2, px, py, pz);
+}
+
 struct NearArgs {
   const float4 *sp;
   const int32_t *cstart, *ccount;
@@ -478,6 +532,7 @@ struct NearArgs {
 // far expansion and writes the base acceleration and the MLP features.
 __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   __shared__ double red[3][3][64];
+  __shared__ float4 tiles[4][64];
   const int item = blockIdx.x;
   if (item >= *a.nitems) return;  // uniform per workgroup
   const int G = a.G;
@@ -486,7 +541,6 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   const int cx = c % G, cy = (c / G) % G, cz = c / (G * G);
   cip cs = (cip)a.cstart;
   cip cc = (cip)a.ccount;
-  cf4p sp4 = (cf4p)a.sp;
   cf4p cr4 = (cf4p)a.crec;
   const int c_begin = cs[c], c_end = cs[c + 1];
   const int t_begin = c_begin + it.y * 64, t_end = min(c_end, t_begin + 64);
@@ -496,6 +550,7 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   const bool tv = k < t_end;
   const float4 me = a.sp[tv ? k : t_begin];
   const float eps2 = to_vgpr(a.eps2);
+  float4 *tl = tiles[wave];
   double ax = 0, ay = 0, az = 0;
   int chunk = 0, nsrc = 0;
   const int xl = max(cx - 1, 0), xh = min(cx + 1, G - 1);
@@ -507,17 +562,19 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
       for (int s0 = r0i; s0 < r1i; s0 += 64, chunk++) {
         if ((chunk & 3) != wave) continue;
         const int cnt = min(64, r1i - s0);
+        // the wave's private tile: one source per lane, zero-mass padding to a multiple of 4
+        tl[lane] = lane < cnt ? a.sp[s0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        wave_lds_sync();
         float px = 0, py = 0, pz = 0;
-        int j = 0;
-        for (; j + 4 <= cnt; j += 4) {  // four sources per scalar fetch
-          const f4v b0 = sp4[s0 + j], b1 = sp4[s0 + j + 1], b2 = sp4[s0 + j + 2],
-                    b3 = sp4[s0 + j + 3];
+        const int cnt4 = (cnt + 3) & ~3;
+        for (int j = 0; j < cnt4; j += 4) {
+          const float4 b0 = tl[j], b1 = tl[j + 1], b2 = tl[j + 2], b3 = tl[j + 3];
           near_pair(b0, me, eps2, px, py, pz);
           near_pair(b1, me, eps2, px, py, pz);
           near_pair(b2, me, eps2, px, py, pz);
           near_pair(b3, me, eps2, px, py, pz);
         }
-        for (; j < cnt; j++) near_pair(sp4[s0 + j], me, eps2, px, py, pz);
+        wave_lds_sync();
         ax += px;
         ay += py;
         az += pz;
@@ -623,53 +680,73 @@ __global__ void k_sur_apply(const uint32_t *perm, const uint32_t *keys2, int n, 
 }
 
 // ---- sampled exact check -------------------------------------------------------------------
-// One workgroup per sampled (sorted) body: FP32 direct sum over all active bodies, threads over
-// strided sources, FP64 tree reduction in fixed order.
-__global__ __launch_bounds__(256) void k_sur_exact(const float4 *sp, const int32_t *cstart,
-                                                   int ncell, const int32_t *sidx, float eps2_s,
-                                                   double *out) {
-  __shared__ double sh[3][256];
-  const int nact = cstart[ncell];
-  const int k = sidx[blockIdx.x];
+// FP32 direct sum for the sampled (sorted) bodies: one wave per (64 samples, 256-source slice),
+// a sample per lane, sources streamed through the scalar cache; FP64 slice partials.
+constexpr int kExactSlice = 256;
+
+__global__ __launch_bounds__(64) void k_sur_exact(const float4 *sp, const int32_t *cstart,
+                                                  int ncell, const int32_t *sidx, int S,
+                                                  float eps2_s, double *part) {
+  __shared__ float4 tl[64];
+  cip cs = (cip)cstart;
+  const int nact = cs[ncell];
+  const int sl = blockIdx.y, j0 = sl * kExactSlice;
+  if (j0 >= nact) return;  // uniform
+  const int j1 = min(j0 + kExactSlice, nact);
+  const int s = blockIdx.x * 64 + threadIdx.x;
+  const bool sv = s < S;
+  const int k = sv ? min(sidx[s], nact - 1) : 0;
+  const float4 me = sp[k];
   const float eps2 = to_vgpr(eps2_s);
+  const int lane = threadIdx.x;
   float px = 0, py = 0, pz = 0;
-  if (k < nact) {
-    const float4 me = sp[k];
-    for (int j = threadIdx.x; j < nact; j += 256) {
-      const float4 b = sp[j];
-      const float dx = b.x - me.x, dy = b.y - me.y, dz = b.z - me.z;
-      float r2 = __builtin_fmaf(dx, dx, eps2);
-      r2 = __builtin_fmaf(dy, dy, r2);
-      r2 = __builtin_fmaf(dz, dz, r2);
-      const float inv = __builtin_amdgcn_rsqf(r2);
-      const float s = b.w * inv * inv * inv;
-      px = __builtin_fmaf(dx, s, px);
-      py = __builtin_fmaf(dy, s, py);
-      pz = __builtin_fmaf(dz, s, pz);
+  for (int jb = j0; jb < j1; jb += 64) {
+    const int cnt = min(64, j1 - jb);
+    tl[lane] = lane < cnt ? sp[jb + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    wave_lds_sync();
+    const int cnt4 = (cnt + 3) & ~3;
+    for (int j = 0; j < cnt4; j += 4) {
+      const float4 b0 = tl[j], b1 = tl[j + 1], b2 = tl[j + 2], b3 = tl[j + 3];
+      near_pair(b0, me, eps2, px, py, pz);
+      near_pair(b1, me, eps2, px, py, pz);
+      near_pair(b2, me, eps2, px, py, pz);
+      near_pair(b3, me, eps2, px, py, pz);
     }
+    wave_lds_sync();
   }
-  sh[0][threadIdx.x] = px;
-  sh[1][threadIdx.x] = py;
-  sh[2][threadIdx.x] = pz;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int q = 0; q < 3; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
-    __syncthreads();
+  if (sv) {
+    double *o = part + ((size_t)sl * S + s) * 3;
+    o[0] = px;
+    o[1] = py;
+    o[2] = pz;
   }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 3; q++) out[(size_t)blockIdx.x * 3 + q] = sh[q][0];
 }
 
-// Relative error of the surrogate over the samples; raises the fallback flag above tol.
-__global__ void k_sur_check(const int32_t *sidx, int S, const uint32_t *perm,
-                            const int32_t *cstart, int ncell, const double *exact,
-                            const double *acc, int npad, double tol, uint32_t *flag,
-                            double *err) {
-  if (threadIdx.x != 0) return;
+// exact[s][q] = sum of the slice partials: one wave per (s, q), lanes over slices (lane l takes
+// slices l, l+64, ... in order), then the butterfly -- a fixed order.
+__global__ __launch_bounds__(256) void k_sur_exact_sum(const double *part, int S,
+                                                       const int32_t *cstart, int ncell,
+                                                       double *exact) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (g >= S * 3) return;  // uniform per wave
+  const int nsl = (cstart[ncell] + kExactSlice - 1) / kExactSlice;
+  double v = 0;
+  for (int sl = lane; sl < nsl; sl += 64) v += part[(size_t)sl * S * 3 + g];
+  v = wsum(v);
+  if (lane == 0) exact[g] = v;
+}
+
+// Relative error of the surrogate over the samples (fixed-order tree reduction); raises the
+// fallback flag above tol.
+__global__ __launch_bounds__(256) void k_sur_check(const int32_t *sidx, int S,
+                                                   const uint32_t *perm, const int32_t *cstart,
+                                                   int ncell, const double *exact,
+                                                   const double *acc, int npad, double tol,
+                                                   uint32_t *flag, double *err) {
+  __shared__ double sn[256], sd[256];
   const int nact = cstart[ncell];
   double num = 0, den = 0;
-  for (int s = 0; s < S; s++) {
+  for (int s = threadIdx.x; s < S; s += 256) {
     const int k = sidx[s];
     if (k >= nact) continue;
     const uint32_t i = perm[k];
@@ -680,9 +757,21 @@ __global__ void k_sur_check(const int32_t *sidx, int S, const uint32_t *perm,
       den += e * e;
     }
   }
-  const double r = den > 0 ? sqrt(num / den) : 0.0;
-  *err = r;
-  *flag = r > tol ? 1u : 0u;
+  sn[threadIdx.x] = num;
+  sd[threadIdx.x] = den;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sn[threadIdx.x] += sn[threadIdx.x + w];
+      sd[threadIdx.x] += sd[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double r = sd[0] > 0 ? sqrt(sn[0] / sd[0]) : 0.0;
+    *err = r;
+    *flag = r > tol ? 1u : 0u;
+  }
 }
 
 // After a fallback force launch: fold the plain kernel's slices into slice 0 (flag set only).
@@ -763,8 +852,9 @@ void sur_free(Sur *s) {
   if (!s) return;
   void *ptrs[] = {s->keys, s->keys2, s->perm_in, s->perm, s->sort_tmp, s->cstart, s->ccount,
                   s->nelist, s->necount, s->sp, s->crec, s->crec_ne, s->fexp, s->part,
-                  s->items, s->nitems, s->feat, s->base, s->center,
+                  s->items, s->nitems, s->btot, s->feat, s->base, s->center,
                   s->red, s->mu, s->sd, s->tx, s->ty, s->sidx, s->sexact, s->serr, s->flag,
+                  s->spart, s->ymlp,
                   s->mu0, s->sd1, s->near_pairs};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -787,6 +877,7 @@ static int sur_ensure_bodies(Ctx *c) {
   NB_HIP(c, salloc(&s->perm, n));
   NB_HIP(c, salloc(&s->sp, n));
   NB_HIP(c, salloc(&s->base, n));
+  NB_HIP(c, salloc(&s->ymlp, n * 3));
   NB_HIP(c, salloc(&s->feat, n * kFeat));
   size_t bytes = 0;
   NB_HIP(c, rocprim::radix_sort_pairs(nullptr, bytes, s->keys, s->keys2, s->perm_in, s->perm,
@@ -802,10 +893,36 @@ static int sur_ensure_bodies(Ctx *c) {
 
 static int sur_ensure_samples(Ctx *c, int S) {
   Sur *s = c->sur;
-  if (S <= s->S) return NB_OK;
-  NB_HIP(c, salloc(&s->sidx, (size_t)S));
-  NB_HIP(c, salloc(&s->sexact, (size_t)S * 3));
-  s->S = S;
+  if (S > s->S) {
+    NB_HIP(c, salloc(&s->sidx, (size_t)S));
+    NB_HIP(c, salloc(&s->sexact, (size_t)S * 3));
+    s->S = S;
+  }
+  const int64_t need = (c->cap + kExactSlice - 1) / kExactSlice * (int64_t)s->S * 3;
+  if (need > s->spart_len) {
+    NB_HIP(c, salloc(&s->spart, (size_t)need));
+    s->spart_len = need;
+  }
+  return NB_OK;
+}
+
+// Sampled error of the accelerations in acc slice 0 against the FP32 direct sum: S bodies evenly
+// spread over the sorted order; the error goes to serr and, above tol, the fallback flag is set.
+static int sur_check_launch(Ctx *c, int S, double tol) {
+  Sur *s = c->sur;
+  hipStream_t st = c->stream;
+  int rc = sur_ensure_samples(c, S);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sur_samples, dim3(nb256(S)), dim3(256), 0, st, s->sidx, S, s->cstart,
+                     s->ncell);
+  const unsigned nsl = (unsigned)((c->n + kExactSlice - 1) / kExactSlice);
+  hipLaunchKernelGGL(k_sur_exact, dim3((unsigned)((S + 63) / 64), nsl), dim3(64), 0, st, s->sp,
+                     s->cstart, s->ncell, s->sidx, S, (float)(c->p.eps * c->p.eps), s->spart);
+  hipLaunchKernelGGL(k_sur_exact_sum, dim3((unsigned)((S * 3 + 3) / 4)), dim3(256), 0, st,
+                     s->spart, S, s->cstart, s->ncell, s->sexact);
+  hipLaunchKernelGGL(k_sur_check, dim3(1), dim3(256), 0, st, s->sidx, S, s->perm, s->cstart,
+                     s->ncell, s->sexact, c->acc, (int)c->npad, tol, s->flag, s->serr);
+  NB_HIP(c, hipGetLastError());
   return NB_OK;
 }
 
@@ -827,10 +944,11 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
   size_t bytes = s->sort_bytes;
   NB_HIP(c, rocprim::radix_sort_pairs(s->sort_tmp, bytes, s->keys, s->keys2, s->perm_in, s->perm,
                                       (size_t)n, 0u, 16u, st));
-  hipLaunchKernelGGL(k_sur_bounds, dim3(nb256(ncell + 1)), dim3(256), 0, st, s->keys2, n, ncell,
-                     s->cstart);
-  hipLaunchKernelGGL(k_sur_scan, dim3(1), dim3(1024), 0, st, s->cstart, ncell, s->ccount,
-                     s->nelist, s->necount, s->items, s->nitems);
+  hipLaunchKernelGGL(k_sur_bounds, dim3((unsigned)(ncell / 1024 + 1)), dim3(1024), 0, st,
+                     s->keys2, n, ncell, s->cstart, s->btot);
+  hipLaunchKernelGGL(k_sur_scan, dim3((unsigned)((ncell + 1023) / 1024)), dim3(1024), 0, st,
+                     s->cstart, ncell, s->btot, s->ccount, s->nelist, s->necount, s->items,
+                     s->nitems);
   hipLaunchKernelGGL(k_sur_gather, dim3(nb256(n)), dim3(256), 0, st, s->perm, s->keys2, n, ncell,
                      c->x, c->y, c->z, c->m, s->center, s->sp);
   hipLaunchKernelGGL(k_sur_moments, dim3((unsigned)ncell), dim3(64), 0, st, s->sp, s->cstart,
@@ -864,15 +982,13 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
   hipLaunchKernelGGL(k_sur_near, dim3(item_ub), dim3(256), 0, st, na);
   NB_HIP(c, hipGetLastError());
   if (!apply) return NB_OK;
-  const float *Y = nullptr;
-  int ldy = 0;
   const bool mlp_on = use_mlp && s->trained && s->mlp;
   if (mlp_on) {
-    rc = mlp_forward_device(s->mlp, s->feat, n, &Y, &ldy, st);
-    if (rc) return fail_msg(c, rc, "surrogate MLP forward failed");
+    rc = mlp_infer_device(s->mlp, s->feat, n, s->ymlp, st);
+    if (rc) return fail_msg(c, rc, "surrogate MLP inference failed");
   }
   hipLaunchKernelGGL(k_sur_apply, dim3(nb256(n)), dim3(256), 0, st, s->perm, s->keys2, n, ncell,
-                     s->base, Y, ldy, s->out_scale, mlp_on ? 1 : 0, c->acc, (int)c->npad);
+                     s->base, s->ymlp, 3, s->out_scale, mlp_on ? 1 : 0, c->acc, (int)c->npad);
   NB_HIP(c, hipGetLastError());
   return NB_OK;
 }
@@ -887,15 +1003,8 @@ int sur_force_phase(Ctx *c, bool timed, int ev_slot) {
   if (rc) return rc;
   if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], st));
   if (s->p.check_every > 0 && s->steps % s->p.check_every == 0 && c->n > 0) {
-    const int S = s->p.check_samples;
-    rc = sur_ensure_samples(c, S);
+    rc = sur_check_launch(c, s->p.check_samples, s->p.tol);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sur_samples, dim3(nb256(S)), dim3(256), 0, st, s->sidx, S, s->cstart,
-                       s->ncell);
-    hipLaunchKernelGGL(k_sur_exact, dim3((unsigned)S), dim3(256), 0, st, s->sp, s->cstart,
-                       s->ncell, s->sidx, (float)(c->p.eps * c->p.eps), s->sexact);
-    hipLaunchKernelGGL(k_sur_check, dim3(1), dim3(64), 0, st, s->sidx, S, s->perm, s->cstart,
-                       s->ncell, s->sexact, c->acc, (int)c->npad, s->p.tol, s->flag, s->serr);
     hipLaunchKernelGGL(k_sur_count_fallback, dim3(1), dim3(64), 0, st, s->flag, c->ctr);
     NB_HIP(c, hipGetLastError());
     // direct-sum redo of this cycle's force when the flag is up (the kernel exits otherwise)
@@ -964,6 +1073,7 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     NB_HIP(c, salloc(&s->necount, 1));
     NB_HIP(c, salloc(&s->crec, nc * kCellRec));
     NB_HIP(c, salloc(&s->crec_ne, nc * kCellRec));
+    NB_HIP(c, salloc(&s->btot, nc / 1024 + 2));
     NB_HIP(c, salloc(&s->fexp, nc * kFarRec));
     NB_HIP(c, salloc(&s->part, nc * kFarRec * (size_t)((nc + kM2LSlice - 1) / kM2LSlice)));
     s->items_cap = 0;  // re-sized for the new grid by sur_ensure_bodies
@@ -1168,16 +1278,8 @@ int nb_sur_error(nb_ctx *c, int32_t samples, int32_t use_mlp, double *err) {
   const bool mlp = use_mlp && s->trained;
   int rc = sur_pipeline(c, false, mlp, true, mlp);
   if (rc) return rc;
-  rc = sur_ensure_samples(c, samples);
+  rc = sur_check_launch(c, samples, 1e300);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_sur_samples, dim3(nb256(samples)), dim3(256), 0, c->stream, s->sidx,
-                     samples, s->cstart, s->ncell);
-  hipLaunchKernelGGL(k_sur_exact, dim3((unsigned)samples), dim3(256), 0, c->stream, s->sp,
-                     s->cstart, s->ncell, s->sidx, (float)(c->p.eps * c->p.eps), s->sexact);
-  hipLaunchKernelGGL(k_sur_check, dim3(1), dim3(64), 0, c->stream, s->sidx, samples, s->perm,
-                     s->cstart, s->ncell, s->sexact, c->acc, (int)c->npad, 1e300, s->flag,
-                     s->serr);
-  NB_HIP(c, hipGetLastError());
   NB_HIP(c, hipMemcpyAsync(err, s->serr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   NB_HIP(c, hipStreamSynchronize(c->stream));
   return NB_OK;

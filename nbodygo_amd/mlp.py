@@ -70,6 +70,13 @@ class MLP:
         self._check(lib.nb_mlp_forward(self._m, _fp(x), x.shape[0], _fp(y)))
         return y
 
+    def infer(self, x):
+        """Fused one-kernel inference (the surrogate's path); equal to forward() bit for bit."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((x.shape[0], self.d_out), np.float32)
+        self._check(lib.nb_mlp_infer(self._m, _fp(x), x.shape[0], _fp(y)))
+        return y
+
     def grad(self, x, t):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)

@@ -64,6 +64,23 @@ def test_device_forward_matches_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("d_in,hidden,layers,rows", [(17, 128, 1, 1000), (30, 128, 3, 4099),
+                                                     (30, 64, 2, 77)])
+def test_fused_inference_equals_layer_by_layer_forward(d_in, hidden, layers, rows):
+    """The surrogate's one-kernel inference reproduces the GEMM-by-GEMM forward bit for bit
+    (same MFMA sequence, same epilogue), including ragged row counts."""
+    from nbodygo_amd.mlp import MLP
+    rng = np.random.default_rng(rows)
+    x = rng.normal(size=(rows, d_in)).astype(np.float32)
+    with MLP(d_in, hidden, layers, D_OUT, seed=rows) as m:
+        np.testing.assert_array_equal(m.infer(x), m.forward(x))
+        x2 = rng.normal(size=(256, d_in)).astype(np.float32)
+        t2 = rng.normal(size=(256, D_OUT)).astype(np.float32)
+        m.train(x2, t2, steps=3, batch=256)  # weights move: the transposed copies follow
+        np.testing.assert_array_equal(m.infer(x), m.forward(x))
+
+
+@pytest.mark.gpu
 def test_device_gradient_matches_oracle():
     from nbodygo_amd.mlp import MLP
     params = random_params(5)
