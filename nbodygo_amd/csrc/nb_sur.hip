@@ -506,6 +506,13 @@ __device__ __forceinline__ void near_pair(f4v b, float4 me, float eps2, float &p
   pz = __builtin_fmaf(dz, s, pz);
 }
 
+__device__ __forceinline__ void quad_field(float4 r0, float4 r1, float4 r2v, float px, float py,
+                                           float pz, float eps2, float &ax, float &ay,
+                                           float &az) {
+  quad_field(f4v{r0.x, r0.y, r0.z, r0.w}, f4v{r1.x, r1.y, r1.z, r1.w},
+             f4v{r2v.x, r2v.y, r2v.z, r2v.w}, px, py, pz, eps2, ax, ay, az);
+}
+
 __device__ __forceinline__ void near_pair(float4 b, float4 me, float eps2, float &px, float &py,
                                           float &pz) {
   near_pair(f4v{b.x, b.y, b.z, b.w}, me, eps2, px, py, pz);
@@ -533,6 +540,7 @@ struct NearArgs {
 __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   __shared__ double red[3][3][64];
   __shared__ float4 tiles[4][64];
+  __shared__ float4 shrec[4][64][3];
   const int item = blockIdx.x;
   if (item >= *a.nitems) return;  // uniform per workgroup
   const int G = a.G;
@@ -540,8 +548,6 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   const int c = it.x;
   const int cx = c % G, cy = (c / G) % G, cz = c / (G * G);
   cip cs = (cip)a.cstart;
-  cip cc = (cip)a.ccount;
-  cf4p cr4 = (cf4p)a.crec;
   const int c_begin = cs[c], c_end = cs[c + 1];
   const int t_begin = c_begin + it.y * 64, t_end = min(c_end, t_begin + 64);
   const int lane = threadIdx.x & 63;
@@ -551,6 +557,7 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   const float4 me = a.sp[tv ? k : t_begin];
   const float eps2 = to_vgpr(a.eps2);
   float4 *tl = tiles[wave];
+  float4 (*rl)[3] = shrec[wave];
   double ax = 0, ay = 0, az = 0;
   int chunk = 0, nsrc = 0;
   const int xl = max(cx - 1, 0), xh = min(cx + 1, G - 1);
@@ -580,24 +587,46 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
         az += pz;
       }
     }
-  // multipole shell, Chebyshev distance 2..K (cells dealt round robin to the waves)
+  // multipole shell, Chebyshev distance 2..K: the nonempty cells of the (2K+1)^3 box, listed by
+  // ballot (one box position per lane, 64 per round) in box order and dealt round robin to the
+  // waves; each wave stages its records in LDS (one per lane) and reads them back as broadcasts.
   {
-    const int K = a.K;
+    const int K = a.K, D = 2 * K + 1, box = D * D * D;
     float sx = 0, sy = 0, sz = 0;
-    int q = 0;
-    for (int z = max(cz - K, 0); z <= min(cz + K, G - 1); z++)
-      for (int y = max(cy - K, 0); y <= min(cy + K, G - 1); y++)
-        for (int x = max(cx - K, 0); x <= min(cx + K, G - 1); x++) {
-          const int ch = max(abs(x - cx), max(abs(y - cy), abs(z - cz)));
-          if (ch < 2) continue;
-          const bool mine = (q & 3) == wave;
-          q++;
-          if (!mine) continue;
-          const int s = x + G * (y + G * z);
-          if (cc[s] == 0) continue;
-          quad_field(cr4[(size_t)s * 4], cr4[(size_t)s * 4 + 1], cr4[(size_t)s * 4 + 2], me.x,
-                     me.y, me.z, eps2, sx, sy, sz);
+    int q = 0;  // list entries before this round
+    for (int p0 = 0; p0 < box; p0 += 64) {
+      const int p = p0 + lane;
+      bool ok = false;
+      int cell = 0;
+      if (p < box) {
+        const int dx = p % D - K, dy = (p / D) % D - K, dz = p / (D * D) - K;
+        const int x = cx + dx, y = cy + dy, z = cz + dz;
+        if (max(abs(dx), max(abs(dy), abs(dz))) >= 2 && x >= 0 && x < G && y >= 0 && y < G &&
+            z >= 0 && z < G) {
+          cell = x + G * (y + G * z);
+          ok = a.ccount[cell] > 0;
         }
+      }
+      const uint64_t okm = __ballot(ok);
+      const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(okm >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)okm, 0u));
+      const bool mine = ok && (((q + rank) & 3) == wave);
+      const uint64_t mm = __ballot(mine);
+      const int slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(mm >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mm, 0u));
+      if (mine) {
+        const float4 *r = (const float4 *)(a.crec + (size_t)cell * kCellRec);
+        rl[slot][0] = r[0];
+        rl[slot][1] = r[1];
+        rl[slot][2] = r[2];
+      }
+      wave_lds_sync();
+      const int nsel = __builtin_popcountll(mm);
+      for (int j = 0; j < nsel; j++)
+        quad_field(rl[j][0], rl[j][1], rl[j][2], me.x, me.y, me.z, eps2, sx, sy, sz);
+      wave_lds_sync();
+      q += __builtin_popcountll(okm);
+    }
     ax += sx;
     ay += sy;
     az += sz;

@@ -25,12 +25,22 @@ _HEX = re.compile(r"^[+-]?0[xX]([0-9a-fA-F]+(\.[0-9a-fA-F]*)?|\.[0-9a-fA-F]+)[pP
 
 
 def go_parse_float(s):
-    """strconv.ParseFloat(s, 64) for the forms a body CSV uses; raises ValueError otherwise."""
-    if _DEC.match(s) or _SPECIAL.match(s):
+    """strconv.ParseFloat(s, 64) for the forms a body CSV uses; raises ValueError otherwise,
+    including for a finite literal beyond the FP64 range (ParseFloat's ErrRange)."""
+    if _SPECIAL.match(s):
         return float(s)
-    if _HEX.match(s):
-        return float.fromhex(s)
-    raise ValueError("invalid float %r" % s)
+    if _DEC.match(s):
+        v = float(s)
+    elif _HEX.match(s):
+        try:
+            v = float.fromhex(s)
+        except OverflowError:
+            raise ValueError("out of range %r" % s) from None
+    else:
+        raise ValueError("invalid float %r" % s)
+    if v in (float("inf"), float("-inf")):
+        raise ValueError("out of range %r" % s)
+    return v
 
 
 def go_parse_bool(s):
