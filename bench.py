@@ -118,6 +118,22 @@ def force_error(g, b, nsample=4096, seed=0):
             "against": "FP64 direct sum on the device (bit-identical to the C oracle in tests)"}
 
 
+def measured_traffic(n, shape):
+    """HBM bytes per force launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (calibrated against a copy kernel), when they were taken at this body count and shape;
+    None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "force_plummer65k_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("n") != n or t.get("shape") != shape:
+        return None
+    return t["bytes_per_launch"]
+
+
 def timed_run(g, b, steps, warmup, dt=1e-3):
     g.load(b)
     if warmup:
@@ -250,7 +266,10 @@ def main():
         "config": {"workload": w["desc"], "n": n, "eps": 0.01, "dt": 1e-3, "G": 1.0,
                    "collisions": "off", "kernel_shape": shape},
         "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": measured_traffic(n, shape),
+                     "traffic_source": "profiles/force_plummer65k_pmc.md (bytes per launch, "
+                                       "null when this run's shape differs)",
                      "kernel": "k_force_scalar/k_force_lds (FP32 pairwise force)",
                      "flops_per_interaction": FLOPS_PER_INTERACTION,
                      "interactions_per_launch": per_launch, "avg_launch_ms": launch_ms},

@@ -156,13 +156,24 @@ __global__ void k_gelu_back(const float *dH, const float *Z, float *dZ, uint16_t
   dZb[i] = f2bf(g);
 }
 
-// Column sums of X[rows][cols] (bias gradients): one thread per column, rows in order.
-__global__ void k_colsum(const float *X, int rows, int cols, float *out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// Column sums of X[rows][cols] (bias gradients): a workgroup of 16 x 64 threads per 64 columns;
+// thread (g, c) sums rows g, g+16, ... in order (coalesced across c), then the 16 partial sums
+// meet in a fixed LDS tree.
+__global__ __launch_bounds__(1024) void k_colsum(const float *X, int rows, int cols,
+                                                 float *out) {
+  __shared__ float sh[16][64];
+  const int cl = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
   float s = 0.f;
-  for (int r = 0; r < rows; r++) s += X[(size_t)r * cols + c];
-  out[c] = s;
+  if (c < cols)
+    for (int r = g; r < rows; r += 16) s += X[(size_t)r * cols + c];
+  sh[g][cl] = s;
+  __syncthreads();
+  for (int w = 8; w > 0; w >>= 1) {
+    if (g < w) sh[g][cl] += sh[g + w][cl];
+    __syncthreads();
+  }
+  if (g == 0 && c < cols) out[c] = sh[0][cl];
 }
 
 // MSE over the first dout columns: L = sum (y - t)^2 / (rows * dout); dY = 2 (y - t) / (rows *
@@ -469,7 +480,7 @@ int backward(nb_mlp *m, int64_t rows) {
   int rc = gemm(m, true, false, inL, m->dYb, m->Wg[L], ldL, m->doutp, R, ldL, m->doutp,
                 EPI_STORE, nullptr, nullptr, splits);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_colsum, dim3(nblk(m->doutp, 64)), dim3(64), 0, m->stream, m->dY, R,
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(m->doutp, 64)), dim3(1024), 0, m->stream, m->dY, R,
                      m->doutp, m->bg[L]);
   NB_MHIP(m, hipGetLastError());
   const uint16_t *gradb = m->dYb;  // BF16 gradient w.r.t. this layer's output
@@ -488,7 +499,7 @@ int backward(nb_mlp *m, int64_t rows) {
     rc = gemm(m, true, false, in, m->dZb, m->Wg[l], ld, m->hid, R, ld, m->hid, EPI_STORE,
               nullptr, nullptr, splits);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_colsum, dim3(nblk(m->hid, 64)), dim3(64), 0, m->stream, m->dZ, R,
+    hipLaunchKernelGGL(k_colsum, dim3(nblk(m->hid, 64)), dim3(1024), 0, m->stream, m->dZ, R,
                        m->hid, m->bg[l]);
     NB_MHIP(m, hipGetLastError());
     gradb = m->dZb;
