@@ -38,6 +38,12 @@ WORKLOADS = {
                                          "dt=1e-3, collisions off: learned far-field surrogate "
                                          "(32^3 stretched grid, exact near field, MLP residual, "
                                          "sampled check every step) against the plain step"),
+    "collide2x524k": dict(n=1048576, desc="two Plummer spheres of 524,288 bodies (masses 1:3, "
+                                          "offset 10a, approaching at 0.5), elastic collisions, "
+                                          "radii 0.05 m^(1/3), eps=0.01, dt=1e-3, 200 steps: "
+                                          "surrogate far field + exact near field (detection "
+                                          "there) against the plain FP32 direct sum; energy and "
+                                          "momentum drift from FP64 diagnostics"),
     "batch512": dict(n=0, desc="512 independent simulations, N uniform in [256, 8192] (seeded), "
                                "uniform-sphere clusters with lognormal masses, eps=0.01, "
                                "dt=1e-3, FP32 force + Kahan, FP64 update, one hipGraph launch "
@@ -94,6 +100,10 @@ def parse():
     ap.add_argument("--sur-sims", type=int, default=200, help="training sims for the surrogate")
     ap.add_argument("--sur-steps", type=int, default=50, help="timed surrogate steps")
     ap.add_argument("--sur-fit-steps", type=int, default=4000, help="Adam steps of the MLP fit")
+    ap.add_argument("--c5-steps", type=int, default=200, help="cycles of the collision run")
+    ap.add_argument("--c5-tol", type=float, default=1e-3,
+                    help="sampled-error tolerance of the surrogate in the collision run")
+    ap.add_argument("--c5-train-sims", type=int, default=8, help="training sims, collision run")
     return ap.parse_args()
 
 
@@ -193,6 +203,78 @@ def surrogate_leg(a, pinned):
     }
 
 
+def run_collide(a, w, pinned):
+    """Config 5: two colliding Plummer spheres, 200 cycles with the surrogate (collisions detected
+    in its exact near field) and 200 with the plain direct sum, from the same start."""
+    from nbodygo_amd import NBodyGPU, gen
+    n_each, steps, dt, tol = 524288, a.c5_steps, 1e-3, a.c5_tol
+    b = gen.two_plummer(n_each, seed=2024, radius_k=0.05)
+
+    def drift(d0, d1):
+        e0 = d0["kinetic"] + d0["potential"]
+        e1 = d1["kinetic"] + d1["potential"]
+        pscale = float(np.sum(b.m * np.linalg.norm(b.vel(), axis=1)))
+        return {"energy_rel": abs(e1 - e0) / abs(e0),
+                "momentum_rel": float(np.linalg.norm(d1["momentum"] - d0["momentum"]) / pscale),
+                "E0": e0, "E1": e1}
+
+    # the plain path runs the LDS-staged kernel shape the autotuner picks at large N
+    g = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=True,
+                 **(pinned or dict(ipl=1, tile_j=512, jsplit=1, block=256, variant=2)))
+    # training rows from seeded sims of the same kind
+    t0 = time.perf_counter()
+    for k in range(a.c5_train_sims):
+        g.load(gen.two_plummer(n_each, seed=300000 + k, radius_k=0.05))
+        g.sur_enable(check_every=0)
+        g.sur_collect(4096, seed=k)
+    loss = g.sur_fit(steps=a.sur_fit_steps, batch=512, lr=1e-3) if a.c5_train_sims else None
+    t_train = time.perf_counter() - t0
+    out = {}
+    for mode in ("surrogate", "plain"):
+        g.load(b)
+        if mode == "surrogate":
+            g.sur_enable(check_every=1, tol=tol)
+            err_mlp = force_error(g, b, nsample=1024)
+            err_base = g.sur_error(1024, use_mlp=False)
+        else:
+            g.sur_disable()
+            err_mlp = force_error(g, b, nsample=1024)
+            err_base = None
+        d0 = g.diagnostics()
+        s0 = g.sur_stats()
+        t0 = time.perf_counter()
+        st = g.step(dt, 1.0, nsteps=steps)
+        el = time.perf_counter() - t0
+        d1 = g.diagnostics()
+        s1 = g.sur_stats()
+        out[mode] = {"ms_per_step": el / steps * 1e3, "steps_per_s": steps / el,
+                     "force_error_start": err_mlp, "pairs_total": st["pairs_total"],
+                     "resolved_total": st["resolved_total"], "drift": drift(d0, d1),
+                     "final": g.store()}
+        if mode == "surrogate":
+            out[mode]["error_without_mlp"] = err_base
+            out[mode]["checks"] = s1["checks"] - s0["checks"]
+            out[mode]["fallbacks"] = s1["fallbacks"] - s0["fallbacks"]
+    g.close()
+    fs, fp = out["surrogate"].pop("final"), out["plain"].pop("final")
+    dpos = float(np.sqrt(np.mean(np.sum((fs.pos() - fp.pos()) ** 2, axis=1))))
+    sur, pl = out["surrogate"], out["plain"]
+    res = {
+        "metric": METRIC, "value": 1e3 / sur["ms_per_step"], "unit": "steps/s (surrogate)",
+        "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": sur["ms_per_step"],
+        "steps_per_s": sur["steps_per_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 force (near field, expansions), bf16 MFMA (MLP), "
+                                        "fp64 update and collisions",
+        "data": "synthetic (seeded Plummer spheres)",
+        "config": {"workload": w["desc"], "n": 2 * n_each, "tolerance": tol},
+        "surrogate": sur, "plain": pl, "speedup_vs_plain": pl["ms_per_step"] / sur["ms_per_step"],
+        "rms_position_difference_after": dpos,
+        "training": {"sims": a.c5_train_sims, "rows_per_sim": 4096, "loss": loss,
+                     "seconds": t_train},
+    }
+    print(json.dumps(res))
+
+
 def cpu_baseline(b, seconds):
     """The reference's worker-pool CPU algorithm (oracle restatement) on a bounded sample: the
     first T targets of the same body set against all N sources, one Compute phase."""
@@ -224,6 +306,8 @@ def main():
     w = WORKLOADS[a.workload]
     if a.workload == "batch512":
         return run_batch(a, w)
+    if a.workload == "collide2x524k":
+        return run_collide(a, w, shape_kw(a.shape))
     if a.workload == "surrogate262k":
         sur = surrogate_leg(a, shape_kw(a.shape))
         n = sur["n"]

@@ -234,11 +234,14 @@ int nb_mlp_train(nb_mlp *m, const float *x, const float *t, int64_t rows, int32_
 /* ---- learned far-field surrogate for the softened force ------------------------------------ */
 /* No counterpart in the reference: its sub-quadratic design, the Barnes-Hut octree
  * (cmd/body/octree.go), is never built.  When enabled on a context (FP32 path, eps > 0, no
- * collisions or overlap cutoff), nb_step and nb_compute_forces replace the O(N^2) force with an
- * exact near field over adjacent cells of a stretched 32^3 grid, per-body quadrupoles of the
- * cells at Chebyshev distance 2..shell, a second-order Taylor far expansion, and an MLP residual;
- * every check_every steps a sampled direct-sum error above tol makes the step fall back to the
- * direct sum.  nb_stats.interactions still counts the N(N-1) pairs the step stands for. */
+ * overlap cutoff), nb_step and nb_compute_forces replace the O(N^2) force with an exact near
+ * field over adjacent cells of a 32^3 grid in stretched coordinates u = d / (|d| + r0) about the
+ * centre of mass, per-body quadrupoles of the cells at Chebyshev distance 2..shell, a
+ * second-order Taylor far expansion, and an MLP residual; every check_every steps a sampled
+ * direct-sum error above tol makes the step fall back to the direct sum.  With collisions on,
+ * detection runs in the near field (exact FP64 test, the same pairs as the direct path); this
+ * needs every radius <= r0 / grid, so that each overlapping pair lies in adjacent cells.
+ * nb_stats.interactions still counts the N(N-1) pairs the step stands for. */
 typedef struct nb_sur_params {
   int32_t grid;          /* cells per axis, 4..32 (32) */
   int32_t shell;         /* per-body multipole shell depth K, 1..6 (2) */

@@ -375,6 +375,10 @@ int nb_load(nb_ctx *c, int64_t n, const double *x, const double *y, const double
   NB_HIP(c, launch_zero_counters(c->ctr, 1, s));
   NB_HIP(c, hipStreamSynchronize(s));
   c->n_active = n ? count_active(n, flags) : 0;
+  c->rmax = 0;
+  for (int64_t i = 0; i < n; i++)
+    if ((flags[i] & NB_F_EXISTS) && !(flags[i] & NB_F_FRAGMENTING) && r[i] > c->rmax)
+      c->rmax = r[i];
   c->loaded = true;
   return NB_OK;
 }

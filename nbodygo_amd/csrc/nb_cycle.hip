@@ -520,26 +520,49 @@ __global__ __launch_bounds__(256) void k_diag_terms(const double *x, const doubl
                                                     const double *vy, const double *vz,
                                                     const double *m, const uint8_t *flags, int n,
                                                     double eps, double *t) {
+  // sources staged 256 at a time in LDS; each body sums its pair terms in ascending j
+  __shared__ double sx[256], sy[256], sz[256], sm[256];
+  __shared__ uint8_t se[256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const bool valid = i < n;
+  const bool ex = valid && (flags[i] & NB_F_EXISTS);
   double ke = 0, pe = 0, px = 0, py = 0, pz = 0, mm = 0, act = 0;
-  if (flags[i] & NB_F_EXISTS) {
-    const double mi = m[i];
+  double xi = 0, yi = 0, zi = 0, mi = 0;
+  if (ex) {
+    mi = m[i];
+    xi = x[i];
+    yi = y[i];
+    zi = z[i];
     ke = 0.5 * mi * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
     px = mi * vx[i];
     py = mi * vy[i];
     pz = mi * vz[i];
     mm = mi;
     act = 1;
-    const double eps2 = eps * eps;
-    for (int j = 0; j < n; j++) {
-      if (j == i || !(flags[j] & NB_F_EXISTS)) continue;
-      const double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
-      const double d2 = dx * dx + dy * dy + dz * dz + eps2;
-      if (d2 > 0) pe -= mi * m[j] / __builtin_sqrt(d2);
-    }
-    pe *= 0.5;
   }
+  const double eps2 = eps * eps;
+  for (int j0 = 0; j0 < n; j0 += 256) {
+    __syncthreads();
+    const int jl = j0 + (int)threadIdx.x;
+    if (jl < n) {
+      sx[threadIdx.x] = x[jl];
+      sy[threadIdx.x] = y[jl];
+      sz[threadIdx.x] = z[jl];
+      sm[threadIdx.x] = m[jl];
+      se[threadIdx.x] = (flags[jl] & NB_F_EXISTS) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!ex) continue;
+    const int cnt = min(256, n - j0);
+    for (int jj = 0; jj < cnt; jj++) {
+      if (j0 + jj == i || !se[jj]) continue;
+      const double dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
+      const double d2 = dx * dx + dy * dy + dz * dz + eps2;
+      if (d2 > 0) pe -= mi * sm[jj] / __builtin_sqrt(d2);
+    }
+  }
+  if (!valid) return;
+  pe *= 0.5;
   double *o = t + (size_t)i * 8;
   o[0] = ke;
   o[1] = pe;

@@ -55,6 +55,7 @@ struct Ctx {
   int64_t npad = 0;  // padded count (multiple of kPad)
   int64_t cap = 0;   // allocated capacity (bodies)
   int64_t n_active = 0;
+  double rmax = 0;  // largest radius of an active body at the last nb_load (radii never change)
   bool loaded = false;
 
   // FP64 state (SoA)

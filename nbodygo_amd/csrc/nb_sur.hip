@@ -87,41 +87,50 @@ __device__ __forceinline__ bool act(uint8_t f) {
   return (f & NB_F_EXISTS) && !(f & NB_F_FRAGMENTING);
 }
 
-// ---- centre of mass (two-stage, fixed order) ------------------------------------------------
+// ---- centre of mass and largest radius (two-stage, fixed order) ------------------------------
 __global__ __launch_bounds__(256) void k_sur_com_part(const double *x, const double *y,
                                                       const double *z, const double *m,
-                                                      const uint8_t *flags, int n, double *part) {
-  __shared__ double sh[4][256];
-  double s[4] = {0, 0, 0, 0};
+                                                      const double *r, const uint8_t *flags,
+                                                      int n, double *part) {
+  __shared__ double sh[5][256];
+  double s[5] = {0, 0, 0, 0, 0};
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
     if (act(flags[i])) {
       s[0] += m[i];
       s[1] += m[i] * x[i];
       s[2] += m[i] * y[i];
       s[3] += m[i] * z[i];
+      s[4] = fmax(s[4], r[i]);
     }
-  for (int c = 0; c < 4; c++) sh[c][threadIdx.x] = s[c];
+  for (int c = 0; c < 5; c++) sh[c][threadIdx.x] = s[c];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
+    if ((int)threadIdx.x < w) {
       for (int c = 0; c < 4; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+      sh[4][threadIdx.x] = fmax(sh[4][threadIdx.x], sh[4][threadIdx.x + w]);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int c = 0; c < 4; c++) part[blockIdx.x * 4 + c] = sh[c][0];
+    for (int c = 0; c < 5; c++) part[blockIdx.x * 5 + c] = sh[c][0];
 }
 
+// center = (COM x, y, z, total mass, largest radius of an active body)
 __global__ __launch_bounds__(256) void k_sur_com_final(const double *part, int nb,
                                                        double *center) {
-  __shared__ double sh[4][256];
-  double s[4] = {0, 0, 0, 0};
-  for (int b = threadIdx.x; b < nb; b += 256)
-    for (int c = 0; c < 4; c++) s[c] += part[b * 4 + c];
-  for (int c = 0; c < 4; c++) sh[c][threadIdx.x] = s[c];
+  __shared__ double sh[5][256];
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    for (int c = 0; c < 4; c++) s[c] += part[b * 5 + c];
+    s[4] = fmax(s[4], part[b * 5 + 4]);
+  }
+  for (int c = 0; c < 5; c++) sh[c][threadIdx.x] = s[c];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
+    if ((int)threadIdx.x < w) {
       for (int c = 0; c < 4; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+      sh[4][threadIdx.x] = fmax(sh[4][threadIdx.x], sh[4][threadIdx.x + w]);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -130,10 +139,15 @@ __global__ __launch_bounds__(256) void k_sur_com_final(const double *part, int n
     center[1] = sh[2][0] / M;
     center[2] = sh[3][0] / M;
     center[3] = sh[0][0];
+    center[4] = sh[4][0];
   }
 }
 
 // ---- binning ---------------------------------------------------------------------------------
+// Stretched grid: cells uniform in u = d / (|d| + r0) around the centre of mass.  The map is
+// 1/r0-Lipschitz (its Jacobian's singular values are 1/(|d|+r0) tangentially and
+// r0/(|d|+r0)^2 radially), so two bodies closer than 2 r0 / G fall in the same or adjacent cells
+// on every axis: with every radius <= r0 / G, each overlapping pair is a near-field pair.
 __device__ __forceinline__ int cell_of(double dx, double dy, double dz, double r0, int G) {
   const double r = sqrt(dx * dx + dy * dy + dz * dz);
   const double s = 1.0 / (r + r0);
@@ -518,6 +532,21 @@ __device__ __forceinline__ void near_pair(float4 b, float4 me, float eps2, float
   near_pair(f4v{b.x, b.y, b.z, b.w}, me, eps2, px, py, pz);
 }
 
+// near_pair that also keeps the smallest r^2 = d^2 + eps^2 seen (collision candidates).
+__device__ __forceinline__ void near_pair_min(float4 b, float4 me, float eps2, float &px,
+                                              float &py, float &pz, float &r2min) {
+  const float dx = b.x - me.x, dy = b.y - me.y, dz = b.z - me.z;
+  float r2 = __builtin_fmaf(dx, dx, eps2);
+  r2 = __builtin_fmaf(dy, dy, r2);
+  r2 = __builtin_fmaf(dz, dz, r2);
+  r2min = fminf(r2min, r2);
+  const float inv = __builtin_amdgcn_rsqf(r2);
+  const float s = b.w * inv * inv * inv;
+  px = __builtin_fmaf(dx, s, px);
+  py = __builtin_fmaf(dy, s, py);
+  pz = __builtin_fmaf(dz, s, pz);
+}
+
 struct NearArgs {
   const float4 *sp;
   const int32_t *cstart, *ccount;
@@ -530,13 +559,64 @@ struct NearArgs {
   int G, K, write_feat;
   float eps2, r0;
   double *near_pairs;
+  // collision detection (near field only; the grid keeps every overlapping pair adjacent)
+  const uint32_t *perm;                // sorted position -> body index
+  const double *x, *y, *z, *r;         // FP64 state
+  const int32_t *beh;
+  uint64_t *pairs;
+  DevCounters *ctr;
+  int64_t pair_cap;
 };
+
+__device__ __forceinline__ float round_up_f(double d) {  // smallest float >= d (d >= 0)
+  float f = (float)d;
+  if ((double)f < d) f = __uint_as_float(__float_as_uint(f) + 1u);
+  return f;
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+__device__ __forceinline__ bool collides(int32_t b) {
+  return b == NB_CB_ELASTIC || b == NB_CB_FRAGMENT;
+}
+
+__device__ __forceinline__ void sur_emit_pair(const NearArgs &a, uint32_t i, uint32_t j) {
+  const uint32_t q = atomicAdd(&a.ctr->pair_count, 1u);
+  if ((int64_t)q < a.pair_cap)
+    a.pairs[q] = ((uint64_t)i << 32) | j;
+  else
+    atomicOr(&a.ctr->pair_overflow, 1u);
+}
+
+// The reference's overlap test, Collided (body.go:192-208), in FP64 with its operation order:
+// sqrt(dx*dx + dy*dy + dz*dz) <= Ri + Rj, for the sources of one chunk against target i; each
+// unordered pair is reported once, by its lower body index.
+__device__ void exact_chunk_pairs(const NearArgs &a, uint32_t i, int s0, int cnt) {
+  const double xi = a.x[i], yi = a.y[i], zi = a.z[i], ri = a.r[i];
+  const bool ci = collides(a.beh[i]);
+  if (!ci) return;
+  for (int jj = 0; jj < cnt; jj++) {
+    const uint32_t j = a.perm[s0 + jj];
+    if (j <= i) continue;
+    const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
+    const double d = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(d > ri + a.r[j]) && collides(a.beh[j])) sur_emit_pair(a, i, j);
+  }
+}
 
 // One workgroup (4 waves) per work item = up to 64 bodies of one cell, one lane each.  The waves
 // split the cell's neighbourhood: 64-source chunks of the nine contiguous rows of adjacent cells
 // (round robin over the chunks) and the shell cells (round robin), streaming both through the
 // scalar cache; partial sums meet in LDS in wave order, and wave 0 adds the Taylor step of the
 // far expansion and writes the base acceleration and the MLP features.
+// COLL: each lane also keeps the minimum r^2 of every chunk and compares it against a bound that
+// no overlapping pair can exceed in FP32 (the chunk's largest radius and coordinate, rounded up,
+// plus the FP32 rounding of positions); a flagged (body, chunk) is re-tested exactly in FP64.
+template <bool COLL>
 __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   __shared__ double red[3][3][64];
   __shared__ float4 tiles[4][64];
@@ -560,6 +640,13 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   float4 (*rl)[3] = shrec[wave];
   double ax = 0, ay = 0, az = 0;
   int chunk = 0, nsrc = 0;
+  uint32_t i_body = 0;
+  float ri = 0.f, me_inf = 0.f;
+  if (COLL && tv) {
+    i_body = a.perm[k];
+    ri = round_up_f(a.r[i_body]);
+    me_inf = fmaxf(fabsf(me.x), fmaxf(fabsf(me.y), fabsf(me.z)));
+  }
   const int xl = max(cx - 1, 0), xh = min(cx + 1, G - 1);
   for (int z = max(cz - 1, 0); z <= min(cz + 1, G - 1); z++)
     for (int y = max(cy - 1, 0); y <= min(cy + 1, G - 1); y++) {
@@ -570,18 +657,59 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
         if ((chunk & 3) != wave) continue;
         const int cnt = min(64, r1i - s0);
         // the wave's private tile: one source per lane, zero-mass padding to a multiple of 4
-        tl[lane] = lane < cnt ? a.sp[s0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-        wave_lds_sync();
-        float px = 0, py = 0, pz = 0;
-        const int cnt4 = (cnt + 3) & ~3;
-        for (int j = 0; j < cnt4; j += 4) {
-          const float4 b0 = tl[j], b1 = tl[j + 1], b2 = tl[j + 2], b3 = tl[j + 3];
-          near_pair(b0, me, eps2, px, py, pz);
-          near_pair(b1, me, eps2, px, py, pz);
-          near_pair(b2, me, eps2, px, py, pz);
-          near_pair(b3, me, eps2, px, py, pz);
+        const float4 src = lane < cnt ? a.sp[s0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        tl[lane] = src;
+        float rmax_c = 0.f, xmax_c = 0.f;
+        if (COLL) {
+          if (lane < cnt) {
+            rmax_c = round_up_f(a.r[a.perm[s0 + lane]]);
+            xmax_c = fmaxf(fabsf(src.x), fmaxf(fabsf(src.y), fabsf(src.z)));
+          }
+          rmax_c = wave_maxf(rmax_c);
+          xmax_c = wave_maxf(xmax_c);
         }
         wave_lds_sync();
+        float px = 0, py = 0, pz = 0, r2min = FLT_MAX;
+        const int cnt4 = (cnt + 3) & ~3;
+        if (COLL) {
+          // the body's own entry (r^2 = eps^2) must not count as a candidate
+          if (s0 < t_end && s0 + cnt > t_begin) {
+            const int self = k - s0;
+            for (int j = 0; j < cnt; j++) {
+              const float keep = r2min;
+              near_pair_min(tl[j], me, eps2, px, py, pz, r2min);
+              if (j == self) r2min = keep;
+            }
+          } else {
+            // zero-mass padding may only add candidates (an exact pass over the real sources)
+            for (int j = 0; j < cnt4; j += 4) {
+              const float4 b0 = tl[j], b1 = tl[j + 1], b2 = tl[j + 2], b3 = tl[j + 3];
+              near_pair_min(b0, me, eps2, px, py, pz, r2min);
+              near_pair_min(b1, me, eps2, px, py, pz, r2min);
+              near_pair_min(b2, me, eps2, px, py, pz, r2min);
+              near_pair_min(b3, me, eps2, px, py, pz, r2min);
+            }
+          }
+        } else {
+          for (int j = 0; j < cnt4; j += 4) {
+            const float4 b0 = tl[j], b1 = tl[j + 1], b2 = tl[j + 2], b3 = tl[j + 3];
+            near_pair(b0, me, eps2, px, py, pz);
+            near_pair(b1, me, eps2, px, py, pz);
+            near_pair(b2, me, eps2, px, py, pz);
+            near_pair(b3, me, eps2, px, py, pz);
+          }
+        }
+        wave_lds_sync();
+        if (COLL && tv) {
+          // |d_fp32 - d| <= e for every pair of this chunk; 2^-20 covers the rounding of both
+          // positions to FP32 and of their differences with room to spare
+          const float e = 9.5367431640625e-7f * (me_inf + xmax_c + ri + rmax_c);
+          const float thr = ri + rmax_c + e;
+          if (r2min <= (thr * thr + eps2) * 1.000002f) {
+            atomicAdd((unsigned long long *)&a.ctr->fixup_tiles, 1ull);
+            exact_chunk_pairs(a, i_body, s0, cnt);
+          }
+        }
         ax += px;
         ay += py;
         az += pz;
@@ -956,7 +1084,9 @@ static int sur_check_launch(Ctx *c, int S, double tol) {
 }
 
 // Binning, moments, far expansion, near field (+ features).  apply: scatter into acc slice 0.
-static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool use_mlp) {
+// detect: collision pairs from the near field into the context's pair list.
+static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool use_mlp,
+                        bool detect = false) {
   Sur *s = c->sur;
   hipStream_t st = c->stream;
   const int n = (int)c->n, G = s->G, ncell = s->ncell;
@@ -964,7 +1094,7 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
   int rc = sur_ensure_bodies(c);
   if (rc) return rc;
   const int nbk = (int)std::min<int64_t>(1024, (n + 255) / 256);
-  hipLaunchKernelGGL(k_sur_com_part, dim3(nbk), dim3(256), 0, st, c->x, c->y, c->z, c->m,
+  hipLaunchKernelGGL(k_sur_com_part, dim3(nbk), dim3(256), 0, st, c->x, c->y, c->z, c->m, c->r,
                      c->flags, n, s->red);
   hipLaunchKernelGGL(k_sur_com_final, dim3(1), dim3(256), 0, st, s->red, nbk, s->center);
   hipLaunchKernelGGL(k_sur_keys, dim3(nb256(n)), dim3(256), 0, st, c->x, c->y, c->z, c->flags,
@@ -1007,8 +1137,20 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
   na.eps2 = eps2;
   na.r0 = (float)s->r0;
   na.near_pairs = s->near_pairs;
+  na.perm = s->perm;
+  na.x = c->x;
+  na.y = c->y;
+  na.z = c->z;
+  na.r = c->r;
+  na.beh = c->beh;
+  na.pairs = c->pairs;
+  na.ctr = c->ctr;
+  na.pair_cap = c->pair_cap;
   const unsigned item_ub = (unsigned)(ncell + (n + 63) / 64 + 1);
-  hipLaunchKernelGGL(k_sur_near, dim3(item_ub), dim3(256), 0, st, na);
+  if (detect)
+    hipLaunchKernelGGL(k_sur_near<true>, dim3(item_ub), dim3(256), 0, st, na);
+  else
+    hipLaunchKernelGGL(k_sur_near<false>, dim3(item_ub), dim3(256), 0, st, na);
   NB_HIP(c, hipGetLastError());
   if (!apply) return NB_OK;
   const bool mlp_on = use_mlp && s->trained && s->mlp;
@@ -1026,9 +1168,13 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
 int sur_force_phase(Ctx *c, bool timed, int ev_slot) {
   Sur *s = c->sur;
   hipStream_t st = c->stream;
+  if (c->p.collisions && !(c->rmax <= s->r0 / s->G * (1.0 - 1e-6)))  // a later nb_load
+    return fail_msg(c, NB_E_UNSUPPORTED,
+                    "collisions with the surrogate need every radius <= r0 / grid");
   NB_HIP(c, launch_zero_counters(c->ctr, 0, st));
   if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot], st));
-  int rc = sur_pipeline(c, false, s->p.use_mlp && s->trained, true, s->p.use_mlp != 0);
+  const bool detect = c->p.collisions != 0;
+  int rc = sur_pipeline(c, false, s->p.use_mlp && s->trained, true, s->p.use_mlp != 0, detect);
   if (rc) return rc;
   if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], st));
   if (s->p.check_every > 0 && s->steps % s->p.check_every == 0 && c->n > 0) {
@@ -1039,12 +1185,15 @@ int sur_force_phase(Ctx *c, bool timed, int ev_slot) {
     // direct-sum redo of this cycle's force when the flag is up (the kernel exits otherwise)
     ForceArgs fa = make_force_args(c);
     fa.run_flag = s->flag;
+    fa.detect = 0;  // the near field already listed every colliding pair
+    fa.drop_flagged = 0;
     NB_HIP(c, launch_force_f32(c->shape, fa, st));
     hipLaunchKernelGGL(k_sur_fold, dim3(nb256(c->n)), dim3(256), 0, st, c->acc, c->shape.jsplit,
                        (int)c->npad, (int)c->n, s->flag);
     NB_HIP(c, hipGetLastError());
     s->checks++;
   }
+  if (detect) NB_HIP(c, launch_sort_pairs(c->pairs, c->pair_cap, c->ctr, st));
   s->steps++;
   return NB_OK;
 }
@@ -1074,9 +1223,8 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
   if (!c->loaded) return fail_msg(c, NB_E_STATE, "nb_sur_enable: load bodies first");
   if (c->p.precision != NB_PREC_FP32 || !(c->p.eps > 0.0))
     return fail_msg(c, NB_E_UNSUPPORTED, "the surrogate needs the FP32 path and eps > 0");
-  if (c->p.collisions || c->p.overlap_cutoff)
-    return fail_msg(c, NB_E_UNSUPPORTED,
-                    "the surrogate path has no collision detection or overlap cutoff");
+  if (c->p.overlap_cutoff)
+    return fail_msg(c, NB_E_UNSUPPORTED, "the surrogate path has no overlap cutoff");
   if (p->grid < 4 || p->grid > kMaxGrid || p->shell < 1 || p->shell > 6 || p->check_every < 0 ||
       (p->check_every > 0 && (p->check_samples < 1 || p->check_samples > 65536)) ||
       !(p->tol > 0) || p->r0 < 0)
@@ -1108,8 +1256,8 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     s->items_cap = 0;  // re-sized for the new grid by sur_ensure_bodies
   }
   if (fresh) {
-    NB_HIP(c, salloc(&s->center, 4));
-    NB_HIP(c, salloc(&s->red, 1024 * 4));
+    NB_HIP(c, salloc(&s->center, 8));
+    NB_HIP(c, salloc(&s->red, 1024 * 5));
     NB_HIP(c, salloc(&s->mu, kFeat));
     NB_HIP(c, salloc(&s->sd, kFeat));
     NB_HIP(c, salloc(&s->mu0, kFeat));
@@ -1169,6 +1317,12 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     }
     s->r0 = 1.5 * std::max(rh, 1e-30);
   }
+  if (c->p.collisions && !(c->rmax <= s->r0 / s->G * (1.0 - 1e-6))) {
+    c->sur_on = false;
+    return fail_msg(c, NB_E_UNSUPPORTED,
+                    "collisions with the surrogate need every radius <= r0 / grid (overlapping "
+                    "pairs must be near-field pairs): use a larger r0 or a coarser grid");
+  }
   c->sur_on = true;
   return sur_ensure_bodies(c);
 }
@@ -1192,6 +1346,8 @@ int nb_sur_collect(nb_ctx *c, int64_t samples, uint64_t seed) {
   if (rc) return rc;
   ForceArgs fa = make_force_args(c);
   fa.run_flag = nullptr;
+  fa.detect = 0;
+  fa.drop_flagged = 0;
   NB_HIP(c, launch_force_f32(c->shape, fa, c->stream));
   int32_t nact = 0;
   NB_HIP(c, hipMemcpyAsync(&nact, s->cstart + s->ncell, 4, hipMemcpyDeviceToHost, c->stream));

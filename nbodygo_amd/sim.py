@@ -191,8 +191,8 @@ class NBodyGPU:
     def sur_enable(self, grid=32, shell=2, r0=0.0, use_mlp=True, check_every=1,
                    check_samples=256, tol=1e-3, seed=12345):
         """Replace the O(N^2) force phase by the cell-grid multipole + MLP surrogate (FP32 path,
-        eps > 0, no collisions / overlap cutoff).  Steps whose sampled error exceeds tol are
-        redone by direct summation."""
+        eps > 0, no overlap cutoff; with collisions every radius must be <= r0 / grid).  Steps
+        whose sampled error exceeds tol are redone by direct summation."""
         p = L.NbSurParams()
         lib.nb_sur_params_init(C.byref(p))
         p.grid, p.shell, p.r0 = int(grid), int(shell), float(r0)

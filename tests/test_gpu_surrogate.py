@@ -114,7 +114,7 @@ def test_check_passes_under_loose_tolerance_and_is_deterministic():
 
 def test_unsupported_configurations_are_rejected():
     b = gen.plummer(2048, seed=8)
-    for kw, code in ((dict(G=1.0, eps=EPS, collisions=True), L.NB_E_UNSUPPORTED),
+    for kw, code in ((dict(G=1.0, eps=EPS, collisions=True), L.NB_E_UNSUPPORTED),  # the gate
                      (dict(G=1.0, eps=EPS, overlap_cutoff=True, collisions=False),
                       L.NB_E_UNSUPPORTED),
                      (dict(G=1.0, eps=0.0, overlap_cutoff=False, collisions=False),
@@ -168,3 +168,73 @@ def test_learned_residual_gain_on_held_out_sims(tmp_path):
         h.sur_load(path)
         assert h.sur_error(2048, use_mlp=True) == trained[1]
         np.testing.assert_array_equal(h.forces(), f_trained)
+
+
+# ---------------------------------------------------------------------------------------------
+# two spheres, and collisions in the near field
+# ---------------------------------------------------------------------------------------------
+def test_two_spheres_error(oracle):
+    b = gen.two_plummer(65536, seed=21)
+    st = oracle.State.from_bodies(b)
+    idx = np.sort(np.random.default_rng(1).choice(b.n, 512, replace=False))
+    ref = oracle.force_soft(st, 1.0, EPS, idx=idx, nthreads=NT)
+    with sim() as g:
+        g.load(b)
+        g.sur_enable(check_every=0, use_mlp=False)
+        err = glob_err(g.forces()[idx], ref)
+        st2 = g.sur_stats()
+    assert err <= 3e-3, err
+    assert st2["nonempty_cells"] > 1000
+
+
+def test_collisions_need_radii_within_a_cell():
+    b = gen.two_plummer(4096, seed=24, radius_k=1.0)
+    with NBodyGPU(G=1.0, eps=EPS, overlap_cutoff=False, collisions=True) as g:
+        g.load(b)
+        with pytest.raises(NBodyError) as e:
+            g.sur_enable(r0=0.5)  # r0 / 32 is below the radii
+        assert e.value.code == L.NB_E_UNSUPPORTED
+        g.sur_enable()
+
+
+@pytest.mark.parametrize("radius_k,mixed", [(0.2, False), (1.0, True)])
+def test_near_field_detection_matches_oracle_pairs(oracle, radius_k, mixed):
+    """Every overlapping pair, exactly the oracle's set (FP64, d <= Ri + Rj), with dense overlaps
+    (about 30,000 pairs at radius_k = 1), removed bodies and every collision behaviour."""
+    b = gen.two_plummer(20000, seed=22, radius_k=radius_k)
+    if mixed:
+        beh = np.random.default_rng(3).integers(0, 4, b.n).astype(np.int32)
+        b.beh[:] = beh
+        b.flags[::13] = 0
+    st = oracle.State.from_bodies(b)
+    with NBodyGPU(G=1.0, eps=EPS, overlap_cutoff=False, collisions=True) as g:
+        g.load(b)
+        g.sur_enable(check_every=0)
+        g.forces()
+        pairs = g.pairs()
+    exp = oracle.detect(st)
+    assert len(exp) > 50
+    np.testing.assert_array_equal(pairs, exp)
+
+
+def test_surrogate_step_with_collisions_resolves_like_the_oracle(oracle):
+    """Detection, the canonical resolution and the collided-skips-kick update: a body that
+    collided this cycle drifts with its resolved velocity only, so its state after the step
+    matches the oracle's to the last ulps of coll3d's transcendentals, whatever the far field."""
+    b = gen.two_plummer(20000, seed=23, radius_k=0.2)
+    st = oracle.State.from_bodies(b)
+    f = oracle.force_soft(st, 1.0, EPS, nthreads=NT)
+    exp_pairs = oracle.detect(st)
+    oracle.resolve(st, exp_pairs)
+    collided = st.collided.astype(bool).copy()
+    oracle.update(st, f, 1e-3, 1.0)
+    with NBodyGPU(G=1.0, eps=EPS, overlap_cutoff=False, collisions=True) as g:
+        g.load(b)
+        g.sur_enable(check_every=0)
+        s = g.step(1e-3, 1.0, 1)
+        out = g.store()
+        pairs = g.pairs()
+    np.testing.assert_array_equal(pairs, exp_pairs)
+    assert s["pairs_last"] == len(exp_pairs) > 20 and collided.sum() > 20
+    np.testing.assert_allclose(out.vel()[collided], st.vel()[collided], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(out.pos()[collided], st.pos()[collided], rtol=1e-13, atol=1e-15)
