@@ -40,7 +40,8 @@ WORKLOADS = {
                                          "sampled check every step) against the plain step"),
     "collide2x524k": dict(n=1048576, desc="two Plummer spheres of 524,288 bodies (masses 1:3, "
                                           "offset 10a, approaching at 0.5), elastic collisions, "
-                                          "radii 0.05 m^(1/3), eps=0.01, dt=1e-3, 200 steps: "
+                                          "radii 0.05 m^(1/3), eps=0.01, dt=1e-3, 200 steps, "
+                                          "40^3 grid, shell 3: "
                                           "surrogate far field + exact near field (detection "
                                           "there) against the plain FP32 direct sum; energy and "
                                           "momentum drift from FP64 diagnostics"),
@@ -104,6 +105,8 @@ def parse():
     ap.add_argument("--c5-tol", type=float, default=1e-3,
                     help="sampled-error tolerance of the surrogate in the collision run")
     ap.add_argument("--c5-train-sims", type=int, default=8, help="training sims, collision run")
+    ap.add_argument("--c5-grid", type=int, default=40, help="surrogate cells per axis, collision run")
+    ap.add_argument("--c5-shell", type=int, default=3, help="surrogate shell depth, collision run")
     return ap.parse_args()
 
 
@@ -225,7 +228,7 @@ def run_collide(a, w, pinned):
     t0 = time.perf_counter()
     for k in range(a.c5_train_sims):
         g.load(gen.two_plummer(n_each, seed=300000 + k, radius_k=0.05))
-        g.sur_enable(check_every=0)
+        g.sur_enable(check_every=0, grid=a.c5_grid, shell=a.c5_shell)
         g.sur_collect(4096, seed=k)
     loss = g.sur_fit(steps=a.sur_fit_steps, batch=512, lr=1e-3) if a.c5_train_sims else None
     t_train = time.perf_counter() - t0
@@ -233,7 +236,7 @@ def run_collide(a, w, pinned):
     for mode in ("surrogate", "plain"):
         g.load(b)
         if mode == "surrogate":
-            g.sur_enable(check_every=1, tol=tol)
+            g.sur_enable(check_every=1, tol=tol, grid=a.c5_grid, shell=a.c5_shell)
             err_mlp = force_error(g, b, nsample=1024)
             err_base = g.sur_error(1024, use_mlp=False)
         else:
@@ -266,7 +269,8 @@ def run_collide(a, w, pinned):
         "vs_baseline": None, "dtype": "fp32 force (near field, expansions), bf16 MFMA (MLP), "
                                         "fp64 update and collisions",
         "data": "synthetic (seeded Plummer spheres)",
-        "config": {"workload": w["desc"], "n": 2 * n_each, "tolerance": tol},
+        "config": {"workload": w["desc"], "n": 2 * n_each, "tolerance": tol,
+                   "surrogate_grid": a.c5_grid, "surrogate_shell": a.c5_shell},
         "surrogate": sur, "plain": pl, "speedup_vs_plain": pl["ms_per_step"] / sur["ms_per_step"],
         "rms_position_difference_after": dpos,
         "training": {"sims": a.c5_train_sims, "rows_per_sim": 4096, "loss": loss,

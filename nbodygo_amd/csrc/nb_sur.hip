@@ -41,7 +41,7 @@ int mlp_train_step_device(nb_mlp *m, const float *x_dev, const float *t_dev, int
 constexpr int kCellRec = 16;  // COM(3), M, Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, radius, ix, iy, iz, n, 0
 constexpr int kFarRec = 20;   // a(3), J(6: xx yy zz xy xz yz), H(10), 0
 constexpr int kFeat = 30;     // MLP input width
-constexpr int kMaxGrid = 32;
+constexpr int kMaxGrid = 40;  // 40^3 + 1 keys fit the 16-bit radix sort
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(4))) f4v *cf4p;  // scalar-cache (uniform) loads

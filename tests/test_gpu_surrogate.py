@@ -128,7 +128,7 @@ def test_unsupported_configurations_are_rejected():
             assert e.value.code == code
     with sim() as g:
         g.load(b)
-        for bad in (dict(grid=2), dict(grid=64), dict(shell=0), dict(tol=0.0), dict(r0=-1.0)):
+        for bad in (dict(grid=2), dict(grid=41), dict(shell=0), dict(tol=0.0), dict(r0=-1.0)):
             with pytest.raises(NBodyError) as e:
                 g.sur_enable(**bad)
             assert e.value.code == L.NB_E_INVALID
