@@ -150,6 +150,11 @@ int nb_get_pairs(nb_ctx *ctx, int32_t *ij, int64_t cap, int64_t *n);
 /* Copy the FP64 state back (any pointer may be NULL to skip it). */
 int nb_store(nb_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
              uint8_t *flags, double *rest);
+/* The whole body state, the inverse of nb_load (after nb_cycle the count is the one it
+ * returned; any pointer may be NULL to skip it). */
+int nb_store_bodies(nb_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy,
+                    double *vz, double *m, double *r, int32_t *beh, uint8_t *flags, double *rest,
+                    double *frag);
 
 /* FP32 positions for rendering (NewRenderable, renderable.go:22-40): xyz is 3*n floats;
  * exists (n bytes, may be NULL) receives the Exists flag. */
@@ -168,6 +173,35 @@ int nb_get_config(nb_ctx *ctx, int32_t *cfg5);
  * (ipl x tile_j x jsplit x block x variant), keep the fastest for this context, and write it to
  * cfg5 (may be NULL).  reps = timed launches per candidate. */
 int nb_autotune(nb_ctx *ctx, int32_t reps, int32_t *cfg5);
+
+/* ---- collection lifecycle on the device ---------------------------------------------------- */
+/* The runner's collection bookkeeping without a round trip of the whole state through the host:
+ * removals, queued adds and compaction (ComputationRunner.processDeletes,
+ * computation-runner.go:176-216; BodyCollection.Cycle, body_collection.go:253-296), and the
+ * GetBody / ModBody handshakes (body_collection.go:111-191) as single-body reads and patches. */
+typedef struct nb_body {
+  double x, y, z, vx, vy, vz, m, r, rest, frag;
+  int32_t beh;
+  uint8_t flags;
+  uint8_t pad[3];
+} nb_body;
+
+/* Queue k bodies (arrays as in nb_load, frag may be NULL) to be appended by the next nb_cycle. */
+int nb_add_bodies(nb_ctx *ctx, int64_t k, const double *x, const double *y, const double *z,
+                  const double *vx, const double *vy, const double *vz, const double *m,
+                  const double *r, const int32_t *beh, const uint8_t *flags, const double *frag);
+/* Cycle: drop the bodies that no longer exist (survivors keep their order), then append the
+ * queued adds, newest first as the reference's event list holds them (body_collection.go:100,
+ * 273-279), each with restitution R.  *n_out = the new body count. */
+int nb_cycle(nb_ctx *ctx, double R, int64_t *n_out);
+/* processDeletes: count > 0 removes bodies at the reference's stride walk over the array
+ * (existing, unpinned; the walk removes one more than asked in some cases, as there); count = -1
+ * removes every existing body; 0 does nothing.  Removed bodies get Exists = 0 and mass 0
+ * (SetNotExists) and stay in the array until nb_cycle.  *removed = bodies removed. */
+int nb_remove_bodies(nb_ctx *ctx, int64_t count, int64_t *removed);
+/* One body's state (GetBody) and a patch of it (ModBody / ApplyMods). */
+int nb_get_body(nb_ctx *ctx, int64_t index, nb_body *out);
+int nb_set_body(nb_ctx *ctx, int64_t index, const nb_body *in);
 
 /* ---- batched path: many independent simulations in one persistent launch ------------------ */
 /* No counterpart in the reference (one BodyCollection per process); for callers that run many

@@ -116,6 +116,18 @@ class NbSurStats(C.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad1"}
 
 
+class NbBody(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "r", "rest",
+                                           "frag")] + [
+        ("beh", C.c_int32),
+        ("flags", C.c_uint8),
+        ("pad", C.c_uint8 * 3),
+    ]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
+
+
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _i32p = C.POINTER(C.c_int32)
@@ -136,11 +148,19 @@ SIGNATURES = [
     ("nb_accel_fp64", C.c_int, [_ctx, _i64p, C.c_int64, _dp, _dp, _dp]),
     ("nb_get_pairs", C.c_int, [_ctx, _i32p, C.c_int64, _i64p]),
     ("nb_store", C.c_int, [_ctx, _dp, _dp, _dp, _dp, _dp, _dp, _u8p, _dp]),
+    ("nb_store_bodies", C.c_int,
+     [_ctx, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _u8p, _dp, _dp]),
     ("nb_store_render", C.c_int, [_ctx, _fp, _u8p]),
     ("nb_diagnostics", C.c_int, [_ctx, C.POINTER(NbDiag)]),
     ("nb_set_timing", C.c_int, [_ctx, C.c_int32]),
     ("nb_get_config", C.c_int, [_ctx, _i32p]),
     ("nb_autotune", C.c_int, [_ctx, C.c_int32, _i32p]),
+    ("nb_add_bodies", C.c_int,
+     [_ctx, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _u8p, _dp]),
+    ("nb_cycle", C.c_int, [_ctx, C.c_double, _i64p]),
+    ("nb_remove_bodies", C.c_int, [_ctx, C.c_int64, _i64p]),
+    ("nb_get_body", C.c_int, [_ctx, C.c_int64, C.POINTER(NbBody)]),
+    ("nb_set_body", C.c_int, [_ctx, C.c_int64, C.POINTER(NbBody)]),
     ("nb_batch_create", C.c_int, [C.POINTER(NbParams), C.POINTER(_ctx)]),
     ("nb_batch_destroy", None, [_ctx]),
     ("nb_batch_last_error", C.c_char_p, [_ctx]),

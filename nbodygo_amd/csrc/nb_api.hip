@@ -7,6 +7,8 @@
 #include <cstring>
 #include <new>
 
+#include <rocprim/rocprim.hpp>
+
 #include "nb_internal.hpp"
 
 using nb::Ctx;
@@ -114,6 +116,34 @@ static int prepare_shape(Ctx *c) {
   }
   NB_HIP(c, launch_rmax_tile(c->r, c->flags, (int)c->n, c->shape.tile_j, tiles, c->rmax_tile,
                              c->stream));
+  return NB_OK;
+}
+
+// Keeps a tuned shape that still fits the padded count; else the default with the params'
+// pinned fields.
+static void choose_shape(Ctx *c) {
+  if (c->shape_tuned && shape_valid(c->shape, c->npad)) return;
+  c->shape = default_shape(c->npad);
+  c->shape_tuned = false;
+  const nb_params &p = c->p;
+  if (p.ipl) c->shape.ipl = p.ipl;
+  if (p.tile_j) c->shape.tile_j = p.tile_j;
+  if (p.jsplit) c->shape.jsplit = p.jsplit;
+  if (p.block) c->shape.block = p.block;
+  if (p.variant) c->shape.variant = p.variant;
+}
+
+// Pair buffer for n bodies: a power of two so the global bitonic network can pad in place.
+static int size_pairs(Ctx *c, int64_t n) {
+  const int64_t pcap =
+      c->p.pair_capacity > 0 ? c->p.pair_capacity : std::max<int64_t>(4 * n, 65536);
+  int64_t pow2 = 1;
+  while (pow2 < pcap) pow2 <<= 1;
+  if (pow2 > c->pair_cap_pow2) {
+    NB_HIP(c, dalloc(&c->pairs, (size_t)pow2));
+    c->pair_cap_pow2 = pow2;
+  }
+  c->pair_cap = pcap;
   return NB_OK;
 }
 
@@ -324,15 +354,10 @@ int nb_load(nb_ctx *c, int64_t n, const double *x, const double *y, const double
     c->diag_len = c->cap * 8 + 64;
     NB_HIP(c, dalloc(&c->diag, c->diag_len));
   }
-  // Pair buffer: a power of two so the global bitonic network can pad in place.
-  int64_t pcap = c->p.pair_capacity > 0 ? c->p.pair_capacity : std::max<int64_t>(4 * n, 65536);
-  int64_t pow2 = 1;
-  while (pow2 < pcap) pow2 <<= 1;
-  if (pow2 > c->pair_cap_pow2) {
-    NB_HIP(c, dalloc(&c->pairs, (size_t)pow2));
-    c->pair_cap_pow2 = pow2;
+  {
+    const int rc = size_pairs(c, n);
+    if (rc) return rc;
   }
-  c->pair_cap = pcap;
   c->n = n;
   c->npad = npad;
   hipStream_t s = c->stream;
@@ -360,16 +385,7 @@ int nb_load(nb_ctx *c, int64_t n, const double *x, const double *y, const double
     NB_HIP(c, hipStreamSynchronize(s));
   }
   NB_HIP(c, launch_write_stream(c->x, c->y, c->z, c->m, c->flags, c->js, (int)n, (int)npad, s));
-  if (!c->shape_tuned || !shape_valid(c->shape, npad)) {
-    c->shape = default_shape(npad);
-    c->shape_tuned = false;
-    const nb_params &p = c->p;
-    if (p.ipl) c->shape.ipl = p.ipl;
-    if (p.tile_j) c->shape.tile_j = p.tile_j;
-    if (p.jsplit) c->shape.jsplit = p.jsplit;
-    if (p.block) c->shape.block = p.block;
-    if (p.variant) c->shape.variant = p.variant;
-  }
+  choose_shape(c);
   int rc = prepare_shape(c);
   if (rc) return rc;
   NB_HIP(c, launch_zero_counters(c->ctr, 1, s));
@@ -577,6 +593,26 @@ int nb_store(nb_ctx *c, double *x, double *y, double *z, double *vx, double *vy,
   return NB_OK;
 }
 
+int nb_store_bodies(nb_ctx *c, double *x, double *y, double *z, double *vx, double *vy,
+                    double *vz, double *m, double *r, int32_t *beh, uint8_t *flags, double *rest,
+                    double *frag) {
+  if (!c) return NB_E_INVALID;
+  if (!c->loaded) return fail(c, NB_E_STATE, "nb_store_bodies: no bodies loaded");
+  if (c->n == 0) return NB_OK;
+  NB_HIP(c, hipSetDevice(c->device));
+  const size_t db = (size_t)c->n * sizeof(double);
+  double *dst[10] = {x, y, z, vx, vy, vz, m, r, rest, frag};
+  const double *src[10] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m, c->r, c->rest, c->frag};
+  for (int k = 0; k < 10; k++)
+    if (dst[k]) NB_HIP(c, hipMemcpyAsync(dst[k], src[k], db, hipMemcpyDeviceToHost, c->stream));
+  if (beh)
+    NB_HIP(c, hipMemcpyAsync(beh, c->beh, (size_t)c->n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+  if (flags) NB_HIP(c, hipMemcpyAsync(flags, c->flags, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
+  return NB_OK;
+}
+
 int nb_store_render(nb_ctx *c, float *xyz, uint8_t *exists) {
   if (!c) return NB_E_INVALID;
   if (!c->loaded) return fail(c, NB_E_STATE, "nb_store_render: no bodies loaded");
@@ -702,6 +738,260 @@ int nb_autotune(nb_ctx *c, int32_t reps, int32_t *cfg) {
   NB_HIP(c, hipStreamSynchronize(c->stream));
   if (cfg) nb_get_config(c, cfg);
   return NB_OK;
+}
+
+}  // extern "C"
+
+// The largest radius over active bodies (the value nb_load computes on the host), from the device
+// arrays after a removal, compaction or patch.
+static int refresh_rmax(Ctx *c) {
+  unsigned long long *d = nullptr, h = 0;
+  NB_HIP(c, hipMalloc((void **)&d, sizeof h));
+  hipError_t e = hipMemsetAsync(d, 0, sizeof h, c->stream);
+  if (e == hipSuccess) e = launch_rmax_active(c->r, c->flags, (int)c->n, d, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  NB_HIP(c, e);
+  double v;
+  memcpy(&v, &h, sizeof v);
+  c->rmax = v;
+  return NB_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Collection lifecycle on the device
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+int nb_add_bodies(nb_ctx *c, int64_t k, const double *x, const double *y, const double *z,
+                  const double *vx, const double *vy, const double *vz, const double *m,
+                  const double *r, const int32_t *beh, const uint8_t *flags, const double *frag) {
+  if (!c) return NB_E_INVALID;
+  if (k < 0 || k > (int64_t)1 << 30) return fail(c, NB_E_INVALID, "nb_add_bodies: bad count");
+  if (k > 0 && (!x || !y || !z || !vx || !vy || !vz || !m || !r || !beh || !flags))
+    return fail(c, NB_E_INVALID, "nb_add_bodies: NULL array");
+  const double *src[10] = {x, y, z, vx, vy, vz, m, r, nullptr, frag};
+  auto &A = c->adds;
+  for (int q = 0; q < 10; q++)
+    for (int64_t i = 0; i < k; i++) A.a[q].push_back(src[q] ? src[q][i] : (q == 8 ? 1.0 : 0.0));
+  A.beh.insert(A.beh.end(), beh, beh + k);
+  A.flags.insert(A.flags.end(), flags, flags + k);
+  return NB_OK;
+}
+
+int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
+  if (!c) return NB_E_INVALID;
+  if (!c->loaded) return fail(c, NB_E_STATE, "nb_cycle: no bodies loaded");
+  if (!std::isfinite(R)) return fail(c, NB_E_INVALID, "nb_cycle: R must be finite");
+  NB_HIP(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const int64_t n = c->n, nadd = c->adds.count();
+  int32_t *keep = nullptr, *pos = nullptr;
+  void *tmp = nullptr;
+  int64_t nk = 0;
+  // one owner for the scratch buffers of this call
+  struct Scratch {
+    int32_t **a, **b;
+    void **t;
+    ~Scratch() {
+      if (*a) (void)hipFree(*a);
+      if (*b) (void)hipFree(*b);
+      if (*t) (void)hipFree(*t);
+    }
+  } scratch{&keep, &pos, &tmp};
+  if (n > 0) {
+    NB_HIP(c, hipMalloc((void **)&keep, (size_t)n * sizeof(int32_t)));
+    NB_HIP(c, hipMalloc((void **)&pos, (size_t)n * sizeof(int32_t)));
+    NB_HIP(c, launch_keep_flags(c->flags, (int)n, keep, st));
+    size_t tb = 0;
+    NB_HIP(c, rocprim::exclusive_scan(nullptr, tb, keep, pos, 0, (size_t)n,
+                                      rocprim::plus<int32_t>(), st));
+    NB_HIP(c, hipMalloc(&tmp, tb ? tb : 1));
+    NB_HIP(c, rocprim::exclusive_scan(tmp, tb, keep, pos, 0, (size_t)n,
+                                      rocprim::plus<int32_t>(), st));
+    int32_t lp = 0, lk = 0;
+    NB_HIP(c, hipMemcpyAsync(&lp, pos + n - 1, 4, hipMemcpyDeviceToHost, st));
+    NB_HIP(c, hipMemcpyAsync(&lk, keep + n - 1, 4, hipMemcpyDeviceToHost, st));
+    NB_HIP(c, hipStreamSynchronize(st));
+    nk = (int64_t)lp + lk;
+  }
+  if (nk == n && nadd == 0) {  // nothing removed, nothing added
+    if (n_out) *n_out = n;
+    return NB_OK;
+  }
+  const int64_t n_new = nk + nadd;
+  if (n_new > (int64_t)1 << 30) return fail(c, NB_E_INVALID, "nb_cycle: too many bodies");
+  const int64_t npad_new = std::max<int64_t>(round_up(n_new, kPad), kPad);
+  const int64_t cap_new = std::max<int64_t>(c->cap, npad_new);
+  double *d[10] = {};
+  int32_t *beh = nullptr;
+  uint8_t *fl = nullptr, *col = nullptr;
+  auto release = [&]() {
+    for (auto *q : d)
+      if (q) (void)hipFree(q);
+    if (beh) (void)hipFree(beh);
+    if (fl) (void)hipFree(fl);
+    if (col) (void)hipFree(col);
+  };
+  bool ok = true;
+  for (int q = 0; q < 10 && ok; q++) ok = hipMalloc((void **)&d[q], (size_t)cap_new * 8) == hipSuccess;
+  ok = ok && hipMalloc((void **)&beh, (size_t)cap_new * 4) == hipSuccess &&
+       hipMalloc((void **)&fl, (size_t)cap_new) == hipSuccess &&
+       hipMalloc((void **)&col, (size_t)cap_new) == hipSuccess;
+  if (!ok) {
+    release();
+    return fail(c, NB_E_NOMEM, "nb_cycle: device allocation failed");
+  }
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < 10 && e == hipSuccess; q++)
+    e = hipMemsetAsync(d[q], 0, (size_t)cap_new * 8, st);
+  if (e == hipSuccess) e = hipMemsetAsync(beh, 0, (size_t)cap_new * 4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(fl, 0, (size_t)cap_new, st);
+  if (e == hipSuccess) e = hipMemsetAsync(col, 0, (size_t)cap_new, st);
+  CompactArgs ca;
+  const double *olds[10] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m, c->r, c->rest, c->frag};
+  for (int q = 0; q < 10; q++) {
+    ca.src[q] = olds[q];
+    ca.dst[q] = d[q];
+  }
+  ca.beh_src = c->beh;
+  ca.beh_dst = beh;
+  ca.flags_src = c->flags;
+  ca.flags_dst = fl;
+  ca.pos = pos;
+  ca.n = (int)n;
+  if (e == hipSuccess) e = launch_compact(ca, st);
+  // queued adds go after the survivors, newest first, each with restitution R: the reference
+  // keeps its events with PushFront (body_collection.go:100) and Cycle walks them from the front
+  // (body_collection.go:273-279, 286-290)
+  auto &A = c->adds;
+  std::fill(A.a[8].begin(), A.a[8].end(), R);
+  for (int q = 0; q < 10; q++) std::reverse(A.a[q].begin(), A.a[q].end());
+  std::reverse(A.beh.begin(), A.beh.end());
+  std::reverse(A.flags.begin(), A.flags.end());
+  for (int q = 0; q < 10 && e == hipSuccess && nadd; q++)
+    e = hipMemcpyAsync(d[q] + nk, A.a[q].data(), (size_t)nadd * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && nadd)
+    e = hipMemcpyAsync(beh + nk, A.beh.data(), (size_t)nadd * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && nadd)
+    e = hipMemcpyAsync(fl + nk, A.flags.data(), (size_t)nadd, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    release();
+    NB_HIP(c, e);
+  }
+  // swap in the new arrays
+  double **cur[10] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->m, &c->r, &c->rest,
+                      &c->frag};
+  for (int q = 0; q < 10; q++) {
+    (void)hipFree(*cur[q]);
+    *cur[q] = d[q];
+  }
+  (void)hipFree(c->beh);
+  (void)hipFree(c->flags);
+  (void)hipFree(c->collided);
+  c->beh = beh;
+  c->flags = fl;
+  c->collided = col;
+  if (cap_new > c->cap) {
+    c->cap = cap_new;
+    NB_HIP(c, dalloc(&c->js, c->cap));
+    c->diag_len = c->cap * 8 + 64;
+    NB_HIP(c, dalloc(&c->diag, c->diag_len));
+    c->acc_split = 0;   // per-slice buffers are re-sized by prepare_shape
+    c->rmax_tile_j = 0;
+  }
+  {
+    const int rc = size_pairs(c, n_new);
+    if (rc) return rc;
+  }
+  c->n = n_new;
+  c->npad = npad_new;
+  NB_HIP(c, launch_write_stream(c->x, c->y, c->z, c->m, c->flags, c->js, (int)n_new,
+                                (int)npad_new, st));
+  choose_shape(c);
+  int rc = prepare_shape(c);
+  if (rc) return rc;
+  for (int64_t i = 0; i < nadd; i++) {
+    const uint8_t f = A.flags[(size_t)i];
+    if ((f & NB_F_EXISTS) && !(f & NB_F_FRAGMENTING)) c->n_active++;
+  }
+  A.clear();
+  rc = refresh_rmax(c);
+  if (rc) return rc;
+  if (n_out) *n_out = n_new;
+  return NB_OK;
+}
+
+int nb_remove_bodies(nb_ctx *c, int64_t count, int64_t *removed) {
+  if (!c) return NB_E_INVALID;
+  if (!c->loaded) return fail(c, NB_E_STATE, "nb_remove_bodies: no bodies loaded");
+  if (count < -1) return fail(c, NB_E_INVALID, "nb_remove_bodies: count < -1");
+  if (removed) *removed = 0;
+  const int64_t n = c->n;
+  if (count == 0 || n == 0) return NB_OK;
+  NB_HIP(c, hipSetDevice(c->device));
+  const bool all = count == -1;
+  const int64_t step = all ? 1 : (count > n ? 1 : n / count);
+  const int64_t npoints = all ? n : (n + step - 1) / step;
+  unsigned long long *cnt = nullptr, h[2] = {0, 0};
+  NB_HIP(c, hipMalloc((void **)&cnt, sizeof h));
+  hipError_t e = hipMemsetAsync(cnt, 0, sizeof h, c->stream);
+  if (e == hipSuccess)
+    e = launch_remove_walk(c->flags, c->m, (int)n, (int)step, (int)npoints, all ? 1 : 0, cnt,
+                           c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = launch_write_stream(c->x, c->y, c->z, c->m, c->flags, c->js, (int)n, (int)c->npad,
+                            c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(cnt);
+  NB_HIP(c, e);
+  c->n_active = std::max<int64_t>(0, c->n_active - (int64_t)h[1]);
+  if (removed) *removed = (int64_t)h[0];
+  return refresh_rmax(c);
+}
+
+int nb_get_body(nb_ctx *c, int64_t index, nb_body *out) {
+  if (!c || !out) return NB_E_INVALID;
+  if (!c->loaded) return fail(c, NB_E_STATE, "nb_get_body: no bodies loaded");
+  if (index < 0 || index >= c->n) return fail(c, NB_E_INVALID, "nb_get_body: index out of range");
+  NB_HIP(c, hipSetDevice(c->device));
+  memset(out, 0, sizeof *out);
+  const double *src[10] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m, c->r, c->rest, c->frag};
+  double *dst[10] = {&out->x, &out->y, &out->z, &out->vx, &out->vy, &out->vz, &out->m, &out->r,
+                     &out->rest, &out->frag};
+  for (int q = 0; q < 10; q++)
+    NB_HIP(c, hipMemcpyAsync(dst[q], src[q] + index, 8, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipMemcpyAsync(&out->beh, c->beh + index, 4, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipMemcpyAsync(&out->flags, c->flags + index, 1, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
+  return NB_OK;
+}
+
+int nb_set_body(nb_ctx *c, int64_t index, const nb_body *in) {
+  if (!c || !in) return NB_E_INVALID;
+  if (!c->loaded) return fail(c, NB_E_STATE, "nb_set_body: no bodies loaded");
+  if (index < 0 || index >= c->n) return fail(c, NB_E_INVALID, "nb_set_body: index out of range");
+  NB_HIP(c, hipSetDevice(c->device));
+  uint8_t old = 0;
+  NB_HIP(c, hipMemcpyAsync(&old, c->flags + index, 1, hipMemcpyDeviceToHost, c->stream));
+  double *dst[10] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m, c->r, c->rest, c->frag};
+  const double *src[10] = {&in->x, &in->y, &in->z, &in->vx, &in->vy, &in->vz, &in->m, &in->r,
+                           &in->rest, &in->frag};
+  for (int q = 0; q < 10; q++)
+    NB_HIP(c, hipMemcpyAsync(dst[q] + index, src[q], 8, hipMemcpyHostToDevice, c->stream));
+  NB_HIP(c, hipMemcpyAsync(c->beh + index, &in->beh, 4, hipMemcpyHostToDevice, c->stream));
+  NB_HIP(c, hipMemcpyAsync(c->flags + index, &in->flags, 1, hipMemcpyHostToDevice, c->stream));
+  NB_HIP(c, launch_write_stream(c->x, c->y, c->z, c->m, c->flags, c->js, (int)c->n,
+                                (int)c->npad, c->stream));
+  NB_HIP(c, launch_rmax_tile(c->r, c->flags, (int)c->n, c->shape.tile_j, tiles_of(c),
+                             c->rmax_tile, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
+  auto active = [](uint8_t f) { return (f & NB_F_EXISTS) && !(f & NB_F_FRAGMENTING); };
+  c->n_active += (active(in->flags) ? 1 : 0) - (active(old) ? 1 : 0);
+  return refresh_rmax(c);
 }
 
 }  // extern "C"

@@ -627,4 +627,102 @@ hipError_t launch_render(const double *x, const double *y, const double *z, cons
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------------
+// Collection lifecycle on the device: compaction (BodyCollection.Cycle) and removals
+// (ComputationRunner.processDeletes).
+// ---------------------------------------------------------------------------------------------
+__global__ void k_keep_flags(const uint8_t *flags, int n, int32_t *keep) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keep[i] = (flags[i] & NB_F_EXISTS) ? 1 : 0;
+}
+
+__global__ void k_compact(CompactArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n || !(a.flags_src[i] & NB_F_EXISTS)) return;
+  const int p = a.pos[i];
+#pragma unroll
+  for (int k = 0; k < 10; k++) a.dst[k][p] = a.src[k][i];
+  a.beh_dst[p] = a.beh_src[i];
+  a.flags_dst[p] = a.flags_src[i];
+}
+
+hipError_t launch_compact(const CompactArgs &a, hipStream_t st) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_keep_flags(const uint8_t *flags, int n, int32_t *keep, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keep_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, flags, n,
+                     keep);
+  return hipGetLastError();
+}
+
+// processDeletes (computation-runner.go:176-216) without its sequential walk: the walk sets
+// shouldRemove at every index that is a multiple of step and clears it at the first existing,
+// unpinned body at or after that point.  So the removed bodies are, for each stride point p, the
+// first such body in [p, p + step) -- or, if there is none, the one the next stride point finds
+// (the flag is still set when the walk gets there).  One thread per stride point scans its own
+// interval; the last one scans to the end.  all: the -1 request, every existing body.
+__global__ void k_remove_walk(uint8_t *flags, double *m, int n, int step, int npoints, int all,
+                              unsigned long long *removed) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npoints) return;
+  int lo, hi;
+  if (all) {
+    lo = k;
+    hi = k + 1;
+  } else {
+    lo = k * step;
+    hi = (k == npoints - 1) ? n : min(n, lo + step);
+  }
+  for (int i = lo; i < hi; i++) {
+    const uint8_t f = flags[i];
+    if ((f & NB_F_EXISTS) && (all || !(f & NB_F_PINNED))) {
+      flags[i] = (uint8_t)(f & ~NB_F_EXISTS);  // SetNotExists (body.go:93-96)
+      m[i] = 0.0;
+      atomicAdd(&removed[0], 1ull);
+      if (!(f & NB_F_FRAGMENTING)) atomicAdd(&removed[1], 1ull);
+      break;
+    }
+  }
+}
+
+hipError_t launch_remove_walk(uint8_t *flags, double *m, int n, int step, int npoints, int all,
+                              unsigned long long *removed, hipStream_t st) {
+  if (npoints <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_remove_walk, dim3((unsigned)((npoints + 255) / 256)), dim3(256), 0, st,
+                     flags, m, n, step, npoints, all, removed);
+  return hipGetLastError();
+}
+
+// Largest radius over active bodies, as the bits of a non-negative double (their unsigned order
+// is the numeric order), for the exact value the host computes when it loads the bodies.
+__global__ __launch_bounds__(256) void k_rmax_active(const double *r, const uint8_t *flags, int n,
+                                                     unsigned long long *out) {
+  __shared__ unsigned long long sh[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (i < n && active_f(flags[i]) && r[i] > 0.0) v = (unsigned long long)__double_as_longlong(r[i]);
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) v = sh[k] > v ? sh[k] : v;
+    if (v) atomicMax(out, v);
+  }
+}
+
+hipError_t launch_rmax_active(const double *r, const uint8_t *flags, int n,
+                              unsigned long long *out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rmax_active, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, r, flags,
+                     n, out);
+  return hipGetLastError();
+}
+
 }  // namespace nb

@@ -55,7 +55,7 @@ struct Ctx {
   int64_t npad = 0;  // padded count (multiple of kPad)
   int64_t cap = 0;   // allocated capacity (bodies)
   int64_t n_active = 0;
-  double rmax = 0;  // largest radius of an active body at the last nb_load (radii never change)
+  double rmax = 0;  // largest radius over active bodies (set by nb_load and the lifecycle calls)
   bool loaded = false;
 
   // FP64 state (SoA)
@@ -92,6 +92,19 @@ struct Ctx {
 
   Sur *sur = nullptr;
   bool sur_on = false;
+
+  // bodies queued by nb_add_bodies, appended by the next nb_cycle
+  struct Adds {
+    std::vector<double> a[10];  // x y z vx vy vz m r rest frag
+    std::vector<int32_t> beh;
+    std::vector<uint8_t> flags;
+    int64_t count() const { return (int64_t)beh.size(); }
+    void clear() {
+      for (auto &v : a) v.clear();
+      beh.clear();
+      flags.clear();
+    }
+  } adds;
 };
 
 bool hip_ok(Ctx *c, hipError_t e, const char *what);
@@ -225,6 +238,24 @@ hipError_t launch_diag(const double *x, const double *y, const double *z, const 
                        hipStream_t st);
 hipError_t launch_render(const double *x, const double *y, const double *z, const uint8_t *flags,
                          int n, float *xyz, hipStream_t st);
+
+// collection lifecycle
+struct CompactArgs {
+  const double *src[10];  // x y z vx vy vz m r rest frag
+  double *dst[10];
+  const int32_t *beh_src;
+  int32_t *beh_dst;
+  const uint8_t *flags_src;
+  uint8_t *flags_dst;
+  const int32_t *pos;  // exclusive prefix sum of the keep flags
+  int n;
+};
+hipError_t launch_keep_flags(const uint8_t *flags, int n, int32_t *keep, hipStream_t st);
+hipError_t launch_compact(const CompactArgs &a, hipStream_t st);
+hipError_t launch_rmax_active(const double *r, const uint8_t *flags, int n,
+                              unsigned long long *out, hipStream_t st);
+hipError_t launch_remove_walk(uint8_t *flags, double *m, int n, int step, int npoints, int all,
+                              unsigned long long *removed, hipStream_t st);
 
 }  // namespace nb
 

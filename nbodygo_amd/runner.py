@@ -3,12 +3,14 @@ queue and computation runner (cmd/body/body_collection.go, cmd/runner/resultqueu
 cmd/runner/computation-runner.go) over libnbgpu.so, with the same phase order as the C++ driver
 in nbodygo_amd/runner/:
 
-    control polls (time scaling, R, deletes, get/mod requests)
-    -> nb_step (force, detection, resolution, update: one cycle on the device)
-    -> renderables into a result queue -> Cycle (compaction of removed bodies, adds)
+    control polls (time scaling, R, deletes -> nb_remove_bodies, get/mod -> nb_get_body /
+    nb_set_body) -> nb_step (force, detection, resolution, update: one cycle on the device)
+    -> renderables into a result queue -> Cycle (nb_add_bodies + nb_cycle: compaction of removed
+    bodies and the queued adds, on the device)
 
-The device holds the body state between cycles; the host copy is pulled (nb_store) only when a
-host-side decision needs it and pushed (nb_load) after the host changed it.
+The device holds the body state for the whole run: it is loaded once, and the host keeps only
+what the device does not (ids, and the radius / sun flag the renderables carry), compacted with
+the Exists flags each render returns.  sync_to_host() pulls the whole state on demand.
 """
 import collections
 import dataclasses
@@ -77,29 +79,11 @@ class ResultQueueHolder:
         return True
 
 
-def remove_indices(exists, pinned, del_cnt):
-    """The bodies processDeletes removes (computation-runner.go:176-216), including the closure
-    quirk: reaching del_cnt returns from one callback only, so the walk goes on and the next
-    stride point removes again."""
-    n = len(exists)
-    out = []
-    if del_cnt == -1:
-        return [i for i in range(n) if exists[i]]
-    if del_cnt <= 0:
-        return out
-    step = 1 if del_cnt > n else n // del_cnt
-    should = False
-    for it in range(n):
-        if it % step == 0:
-            should = True
-        if should and not pinned[it] and exists[it]:
-            out.append(it)
-            should = False
-    return out
-
-
 class BodyCollection:
-    """The host copy of the sim's bodies (body_collection.go) as a Bodies SoA plus ids."""
+    """The host side of the sim's bodies (body_collection.go): a Bodies SoA mirror plus ids.
+
+    Index i here is index i on the device.  The physical state in `b` is as of the last
+    sync_to_host(); ids, radii and flags are kept current by the runner."""
 
     def __init__(self, bodies: Bodies, ids=None):
         self.b = bodies.copy()
@@ -112,25 +96,35 @@ class BodyCollection:
         return self.b.n
 
     def enqueue_add(self, body: Bodies):
-        self._adds.append(body.copy())
+        """Queues body for the next Cycle; its ids are drawn now, as NewBody draws them."""
+        self._adds.append((body.copy(), np.array([next_id() for _ in range(body.n)], np.int64)))
 
     def has_adds(self):
         return bool(self._adds)
 
-    def cycle(self, R):
-        """Compacts away removed bodies and appends queued adds (each gets r = R)
-        (body_collection.go:253-296).  Returns True when the array changed."""
+    def take_adds(self):
+        """The queued adds as (Bodies, ids) in enqueue order; the queue is emptied."""
+        adds, self._adds = self._adds, []
+        return adds
+
+    def cycle(self, R, keep=None, adds=()):
+        """Mirrors nb_cycle: survivors (keep, default the Exists flags) in order, then the adds
+        ((Bodies, ids) in enqueue order, or Bodies, whose ids are drawn here) newest first with
+        r = R (body_collection.go:253-296).  Returns True when the array changed."""
         from .gen import concat
-        keep = (self.b.flags & L.NB_F_EXISTS).astype(bool)
-        changed = not keep.all() or bool(self._adds)
+        if keep is None:
+            keep = (self.b.flags & L.NB_F_EXISTS).astype(bool)
+        adds = [a if isinstance(a, tuple) else
+                (a, np.array([next_id() for _ in range(a.n)], np.int64)) for a in adds]
+        changed = not keep.all() or bool(adds)
         if not keep.all():
             self.b = _select(self.b, keep)
             self.ids = self.ids[keep]
-        for a in self._adds:
+        for a, ids in reversed(adds):
+            a = _select(a, np.arange(a.n)[::-1])
             a.rest[:] = R
             self.b = concat(self.b, a)
-            self.ids = np.concatenate([self.ids, [next_id() for _ in range(a.n)]])
-        self._adds = []
+            self.ids = np.concatenate([self.ids, ids[::-1]])
         self.cycle_count += 1
         return changed
 
@@ -156,6 +150,7 @@ class ComputationRunner:
         self._R_box = None
         self._del_box = None
         self._dirty = True
+        self._loaded = False
         self._stale = False
         self.stats = collections.Counter()
 
@@ -173,51 +168,60 @@ class ComputationRunner:
         self._del_box = int(n)
 
     def sync_to_host(self):
-        if self._stale:
+        """Pulls the whole device state into bc.b."""
+        if self._stale and self._loaded and self.bc.count():
             s = self.sim.store()
-            for k in ("x", "y", "z", "vx", "vy", "vz", "flags", "rest"):
-                getattr(self.bc.b, k)[:] = getattr(s, k)
-            gone = ~(self.bc.b.flags & L.NB_F_EXISTS).astype(bool)
-            self.bc.b.m[gone] = 0.0
-            self._stale = False
+            assert s.n == self.bc.count()
+            self.bc.b = s
+        self._stale = False
+
+    def _index(self, id_):
+        hit = np.nonzero(self.bc.ids == id_)[0]
+        return hit
 
     def get_body(self, id_):
-        self.sync_to_host()
-        hit = np.nonzero(self.bc.ids == id_)[0]
+        """GetBody (body_collection.go:111-151): one body read from the device, or None."""
+        self._push_if_dirty()
+        hit = self._index(id_)
         if hit.size == 0:
             return None
-        return _select(self.bc.b, np.isin(np.arange(self.bc.count()), hit[:1]))
+        d = self.sim.get_body(int(hit[0]))
+        return Bodies(*([d[k]] for k in ("x", "y", "z", "vx", "vy", "vz", "m", "r")),
+                      beh=[d["beh"]], flags=[d["flags"]], rest=[d["rest"]], frag=[d["frag"]])
 
     def mod_body(self, id_, **fields):
-        self.sync_to_host()
-        hit = np.nonzero(self.bc.ids == id_)[0]
-        for k, v in fields.items():
-            getattr(self.bc.b, k)[hit] = v
+        """ModBody (body_collection.go:154-191): the fields patched on the device."""
+        self._push_if_dirty()
+        hit = self._index(id_)
+        for i in hit:
+            self.sim.set_body(int(i), **fields)
+            for k in ("r", "flags"):  # what the renderables read from the host side
+                if k in fields:
+                    getattr(self.bc.b, k)[i] = fields[k]
         if hit.size:
-            self._dirty = True
+            self._stale = True
         return int(hit.size)
 
     def _process_deletes(self):
+        """processDeletes (computation-runner.go:176-216) as nb_remove_bodies."""
         if self._del_box is None:
             return 0
         d, self._del_box = self._del_box, None
-        self.sync_to_host()
-        b = self.bc.b
-        idx = remove_indices((b.flags & L.NB_F_EXISTS).astype(bool),
-                             (b.flags & L.NB_F_PINNED).astype(bool), d)
-        for i in idx:  # SetNotExists (body.go:93-96)
-            b.flags[i] &= ~np.uint8(L.NB_F_EXISTS)
-            b.m[i] = 0.0
-        if idx:
-            self._dirty = True
-        return len(idx)
+        self._push_if_dirty()
+        if self.bc.count() == 0:
+            return 0
+        k = self.sim.remove_bodies(d)
+        if k:
+            self._stale = True
+        self.stats["removed"] += k
+        return k
 
     def _push_if_dirty(self):
-        if self._dirty and self.bc.count() > 0:
+        if self._dirty:
             self.sim.load(self.bc.b)
+            self._loaded = True
             self.stats["loads"] += 1
         self._dirty = False
-        self._stale = False
 
     def run_one_computation(self):
         """One cycle in the reference's phase order (computation-runner.go:267-326)."""
@@ -247,10 +251,16 @@ class ComputationRunner:
         else:
             gone = False
         self.rqh.add(rq)
-        if gone or self.bc.has_adds():
-            self.sync_to_host()
-        if self.bc.cycle(self.R):
-            self._dirty = True
+        adds = self.bc.take_adds()
+        if gone or adds:  # Cycle on the device, mirrored on the host side
+            for a, _ in adds:
+                self.sim.add_bodies(a)
+            self.sim.cycle(self.R)
+            self.bc.cycle(self.R, ex.astype(bool) if n else None, adds)
+            self._stale = True
+            self.stats["compactions"] += 1
+        else:
+            self.bc.cycle_count += 1
         self.stats["computations"] += 1
         self.stats["wall_ms"] += (time.perf_counter() - t0) * 1e3
         return st

@@ -317,26 +317,30 @@ void BodyCollection::EnqueueAdd(const Body &b) {
   std::lock_guard<std::mutex> g(lock_);
   adds_.push_back(b);
 }
-bool BodyCollection::Cycle(double R) {
+std::vector<Body> BodyCollection::TakeAdds() {
   std::lock_guard<std::mutex> g(lock_);
-  const size_t live = (size_t)std::count_if(arr_.begin(), arr_.end(),
-                                            [](const Body &b) { return b.exists; });
+  std::vector<Body> a;
+  a.swap(adds_);
+  return a;
+}
+bool BodyCollection::Cycle(double R, const uint8_t *keep, const std::vector<Body> &adds) {
+  std::lock_guard<std::mutex> g(lock_);
   bool changed = false;
-  if (live < arr_.size()) {
-    std::vector<Body> a;
-    a.reserve(live + adds_.size());
-    for (auto &b : arr_)
-      if (b.exists) a.push_back(b);
-    arr_.swap(a);
-    changed = true;
+  std::vector<Body> a;
+  a.reserve(arr_.size() + adds.size());
+  for (size_t i = 0; i < arr_.size(); i++) {
+    if (keep ? keep[i] != 0 : arr_[i].exists)
+      a.push_back(arr_[i]);
+    else
+      changed = true;
   }
-  for (auto &b : adds_) {
-    Body c = b;
+  for (auto it = adds.rbegin(); it != adds.rend(); ++it) {  // newest first, as nb_cycle
+    Body c = *it;
     c.r = R;
-    arr_.push_back(c);
+    a.push_back(c);
     changed = true;
   }
-  adds_.clear();
+  if (changed) arr_.swap(a);
   cycle_++;
   return changed;
 }
@@ -400,48 +404,68 @@ void ComputationRunner::run() {
   req_cv_.notify_all();
 }
 
+nb_body ComputationRunner::ToDevice(const Body &b) {
+  nb_body d{};
+  d.x = b.x, d.y = b.y, d.z = b.z, d.vx = b.vx, d.vy = b.vy, d.vz = b.vz;
+  d.m = b.mass, d.r = b.radius, d.rest = b.r, d.frag = b.frag_factor;
+  d.beh = b.collision_behavior;
+  d.flags = (uint8_t)((b.exists ? NB_F_EXISTS : 0) | (b.fragmenting ? NB_F_FRAGMENTING : 0) |
+                      (b.is_sun ? NB_F_SUN : 0) | (b.pinned ? NB_F_PINNED : 0));
+  return d;
+}
+
+void ComputationRunner::FromDevice(const nb_body &d, Body *b) {
+  b->x = d.x, b->y = d.y, b->z = d.z, b->vx = d.vx, b->vy = d.vy, b->vz = d.vz;
+  b->mass = d.m, b->radius = d.r, b->r = d.rest, b->frag_factor = d.frag;
+  b->collision_behavior = d.beh;
+  b->exists = (d.flags & NB_F_EXISTS) != 0;
+  b->fragmenting = (d.flags & NB_F_FRAGMENTING) != 0;
+  b->is_sun = (d.flags & NB_F_SUN) != 0;
+  b->pinned = (d.flags & NB_F_PINNED) != 0;
+}
+
 int ComputationRunner::SyncToHost() {
   if (!host_stale_) return NB_OK;
   auto &arr = bc_->GetArray();
   const size_t n = arr.size();
-  std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), rest(n);
-  std::vector<uint8_t> fl(n);
-  const int rc = nb_store(ctx_, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
-                          fl.data(), rest.data());
-  if (rc != NB_OK) {
-    err_ = nb_last_error(ctx_);
-    return rc;
-  }
-  for (size_t i = 0; i < n; i++) {
-    Body &b = arr[i];
-    b.x = x[i], b.y = y[i], b.z = z[i], b.vx = vx[i], b.vy = vy[i], b.vz = vz[i], b.r = rest[i];
-    const bool ex = (fl[i] & NB_F_EXISTS) != 0;
-    if (!ex && b.exists) b.SetNotExists();
-    b.fragmenting = (fl[i] & NB_F_FRAGMENTING) != 0;
-  }
-  host_stale_ = false;
-  return NB_OK;
-}
-
-int ComputationRunner::pushIfDirty() {
-  if (!dirty_) return NB_OK;
-  auto &arr = bc_->GetArray();
-  const size_t n = arr.size();
-  if (n == 0) {  // nothing to hold on the device; nb_step is not called for an empty collection
-    dirty_ = false;
+  if (n == 0) {
     host_stale_ = false;
     return NB_OK;
   }
   std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), m(n), r(n), rest(n), frag(n);
   std::vector<int32_t> beh(n);
   std::vector<uint8_t> fl(n);
+  const int rc = nb_store_bodies(ctx_, x.data(), y.data(), z.data(), vx.data(), vy.data(),
+                                 vz.data(), m.data(), r.data(), beh.data(), fl.data(),
+                                 rest.data(), frag.data());
+  if (rc != NB_OK) {
+    err_ = nb_last_error(ctx_);
+    return rc;
+  }
   for (size_t i = 0; i < n; i++) {
-    const Body &b = arr[i];
-    x[i] = b.x, y[i] = b.y, z[i] = b.z, vx[i] = b.vx, vy[i] = b.vy, vz[i] = b.vz;
-    m[i] = b.mass, r[i] = b.radius, rest[i] = b.r, frag[i] = b.frag_factor;
-    beh[i] = b.collision_behavior;
-    fl[i] = (uint8_t)((b.exists ? NB_F_EXISTS : 0) | (b.fragmenting ? NB_F_FRAGMENTING : 0) |
-                      (b.is_sun ? NB_F_SUN : 0) | (b.pinned ? NB_F_PINNED : 0));
+    nb_body d;
+    d.x = x[i], d.y = y[i], d.z = z[i], d.vx = vx[i], d.vy = vy[i], d.vz = vz[i];
+    d.m = m[i], d.r = r[i], d.rest = rest[i], d.frag = frag[i], d.beh = beh[i], d.flags = fl[i];
+    FromDevice(d, &arr[i]);
+  }
+  host_stale_ = false;
+  return NB_OK;
+}
+
+// The one nb_load of a run (an empty collection loads too, so later adds have a context).
+int ComputationRunner::pushIfDirty() {
+  if (!dirty_) return NB_OK;
+  auto &arr = bc_->GetArray();
+  const size_t n = arr.size();
+  std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), m(n), r(n), rest(n), frag(n);
+  std::vector<int32_t> beh(n);
+  std::vector<uint8_t> fl(n);
+  for (size_t i = 0; i < n; i++) {
+    const nb_body d = ToDevice(arr[i]);
+    x[i] = d.x, y[i] = d.y, z[i] = d.z, vx[i] = d.vx, vy[i] = d.vy, vz[i] = d.vz;
+    m[i] = d.m, r[i] = d.r, rest[i] = d.rest, frag[i] = d.frag;
+    beh[i] = d.beh;
+    fl[i] = d.flags;
   }
   const int rc = nb_load(ctx_, (int64_t)n, x.data(), y.data(), z.data(), vx.data(), vy.data(),
                          vz.data(), m.data(), r.data(), beh.data(), fl.data(), rest.data(),
@@ -457,39 +481,22 @@ int ComputationRunner::pushIfDirty() {
   return NB_OK;
 }
 
-// computation-runner.go:176-216, including its closure-return quirk: reaching the requested
-// count returns from one callback only, so the walk goes on and the next stride point removes
-// again.
+// processDeletes (computation-runner.go:176-216) on the device: nb_remove_bodies walks the
+// array as the reference does, closure-return quirk included (reaching the requested count
+// returns from one callback only, so the walk goes on and the next stride point removes again).
 void ComputationRunner::processDeletes() {
   const auto d = del_box_.Poll();
   if (!d || *d == 0) return;
-  if (SyncToHost() != NB_OK) return;
-  const int del_cnt = *d;
-  int removed = 0;
-  auto &arr = bc_->GetArray();
-  if (del_cnt == -1) {
-    for (auto &b : arr)
-      if (b.exists) {
-        b.SetNotExists();
-        removed++;
-      }
-  } else if (del_cnt > 0) {
-    const int count = (int)arr.size();
-    const int step = del_cnt > count ? 1 : count / del_cnt;
-    bool should_remove = false;
-    int iter = 0;
-    for (auto &b : arr) {
-      if (iter % step == 0) should_remove = true;
-      iter++;
-      if (should_remove && !b.pinned && b.exists) {
-        b.SetNotExists();
-        should_remove = false;
-        removed++;
-      }
-    }
+  if (pushIfDirty() != NB_OK) return;
+  int64_t removed = 0;
+  if (!bc_->GetArray().empty() && nb_remove_bodies(ctx_, *d, &removed) != NB_OK) {
+    err_ = nb_last_error(ctx_);
+    std::fprintf(stderr, "[ERROR] computation runner: %s\n", err_.c_str());
+    return;
   }
-  if (removed) dirty_ = true;
-  std::fprintf(stderr, "[INFO] Computation runner set %d bodies to not exist\n", removed);
+  if (removed) host_stale_ = true;
+  std::fprintf(stderr, "[INFO] Computation runner set %lld bodies to not exist\n",
+               (long long)removed);
 }
 
 void ComputationRunner::handleRequests() {
@@ -499,25 +506,49 @@ void ComputationRunner::handleRequests() {
     todo.swap(requests_);
   }
   if (todo.empty()) return;
-  const bool synced = SyncToHost() == NB_OK;
+  const bool ready = pushIfDirty() == NB_OK;
   for (Request *r : todo) {
-    if (synced) {
+    if (ready) {
       auto &arr = bc_->GetArray();
-      if (r->kind == 1) {
-        for (auto &b : arr)
+      if (r->kind == 1) {  // HandleGetBody: one body read from the device
+        for (size_t i = 0; i < arr.size(); i++) {
+          const Body &b = arr[i];
           if ((!r->name.empty() && r->name == b.name) || r->id == b.id) {
-            r->body = b;
+            nb_body d;
+            if (nb_get_body(ctx_, (int64_t)i, &d) != NB_OK) {
+              err_ = nb_last_error(ctx_);
+              break;
+            }
+            Body out = b;
+            FromDevice(d, &out);
+            r->body = out;
             break;
           }
-      } else {
+        }
+      } else {  // HandleModBody: each match read, modified and written back on the device
         int found = 0, modified = 0;
-        for (auto &b : arr)
+        for (size_t i = 0; i < arr.size(); i++) {
+          Body &b = arr[i];
           if ((!r->cls.empty() && r->cls == b.cls) || (!r->name.empty() && r->name == b.name) ||
               r->id == b.id) {
             found++;
-            if (b.ApplyMods(r->mods)) modified++;
+            nb_body d;
+            if (nb_get_body(ctx_, (int64_t)i, &d) != NB_OK) {
+              err_ = nb_last_error(ctx_);
+              continue;
+            }
+            Body t = b;
+            FromDevice(d, &t);
+            if (!t.ApplyMods(r->mods)) continue;
+            const nb_body nd = ToDevice(t);
+            if (nb_set_body(ctx_, (int64_t)i, &nd) != NB_OK) {
+              err_ = nb_last_error(ctx_);
+              continue;
+            }
+            b = t;
+            modified++;
           }
-        if (modified) dirty_ = true;
+        }
         r->result = found == 0 ? 0 : modified == 0 ? 1 : found == modified ? 3 : 2;
       }
     }
@@ -629,12 +660,35 @@ int ComputationRunner::cycle() {
     rq->Add(r);
   }
   rqh_->Add(std::move(*rq));
-  // Cycle: compaction and adds work on the host copy, so pull the advanced state first
-  if (gone || bc_->HasAdds()) {
-    rc = SyncToHost();
-    if (rc != NB_OK) return rc;
+  // Cycle: compaction and adds on the device, the host collection compacted alongside
+  std::vector<Body> adds = bc_->TakeAdds();
+  if (gone || !adds.empty()) {
+    const size_t k = adds.size();
+    std::vector<double> ax(k), ay(k), az(k), avx(k), avy(k), avz(k), am(k), ar(k), af(k);
+    std::vector<int32_t> abeh(k);
+    std::vector<uint8_t> afl(k);
+    for (size_t i = 0; i < k; i++) {
+      const nb_body d = ToDevice(adds[i]);
+      ax[i] = d.x, ay[i] = d.y, az[i] = d.z, avx[i] = d.vx, avy[i] = d.vy, avz[i] = d.vz;
+      am[i] = d.m, ar[i] = d.r, af[i] = d.frag, abeh[i] = d.beh, afl[i] = d.flags;
+    }
+    int64_t n_new = 0;
+    rc = nb_add_bodies(ctx_, (int64_t)k, ax.data(), ay.data(), az.data(), avx.data(), avy.data(),
+                       avz.data(), am.data(), ar.data(), abeh.data(), afl.data(), af.data());
+    if (rc == NB_OK) rc = nb_cycle(ctx_, R_, &n_new);
+    if (rc != NB_OK) {
+      err_ = nb_last_error(ctx_);
+      return rc;
+    }
+    bc_->Cycle(R_, n ? ex_.data() : nullptr, adds);
+    if ((size_t)n_new != bc_->GetArray().size()) {
+      err_ = "device and host collections disagree on the body count after Cycle";
+      return NB_E_STATE;
+    }
+    host_stale_ = true;
+  } else {
+    bc_->Cycle(R_, nullptr, adds);
   }
-  if (bc_->Cycle(R_)) dirty_ = true;
   std::lock_guard<std::mutex> g(stats_m_);
   st_.computations++;
   st_.step_ms += s.wall_ms;

@@ -2,13 +2,15 @@
 // result queue and computation runner (cmd/body/body_collection.go, cmd/runner/resultqueue.go,
 // cmd/runner/computation-runner.go), driving libnbgpu.so through include/nbody_gpu.h.
 //
-// The device holds the authoritative body state between cycles.  The host collection mirrors it
-// only where the runner needs host-side decisions: adds, deletes, body queries and mods
-// (pulled with nb_store, pushed with nb_load), and the compaction of bodies that stopped existing
-// (the reference's Cycle, body_collection.go:253-296).  Phase order per cycle is that of
-// runOneComputation (computation-runner.go:267-326): control polls, force + detection +
-// resolution + update on the device (nb_step, one cycle), renderables into a result queue,
-// Cycle.
+// The device holds the body state for the whole run: it is loaded once (nb_load) and every
+// later change happens there -- deletes (nb_remove_bodies), body queries and mods (nb_get_body,
+// nb_set_body), and Cycle's compaction and adds (nb_add_bodies + nb_cycle,
+// body_collection.go:253-296).  The host collection keeps what the device does not (ids, names,
+// classes, colours, intensities, frag steps) index for index with the device, compacted with the
+// Exists flags each render returns; its physical fields are as of the last SyncToHost (pulled
+// with nb_store_bodies).  Phase order per cycle is that of runOneComputation
+// (computation-runner.go:267-326): control polls, force + detection + resolution + update on the
+// device (nb_step, one cycle), renderables into a result queue, Cycle.
 #pragma once
 
 #include <atomic>
@@ -112,9 +114,11 @@ class BodyCollection {
     std::lock_guard<std::mutex> g(lock_);
     return !adds_.empty();
   }
-  // Compacts away bodies that stopped existing and appends queued adds (each gets r = R).
-  // Returns true when the array changed.
-  bool Cycle(double R);
+  std::vector<Body> TakeAdds();
+  // The host half of nb_cycle: keeps the bodies whose keep flag is set (nullptr: the exists
+  // field), then appends adds newest first, each with r = R.  Returns true when the array
+  // changed.
+  bool Cycle(double R, const uint8_t *keep, const std::vector<Body> &adds);
   int cycle() const { return cycle_; }
 
  private:
@@ -187,8 +191,11 @@ class ComputationRunner {
 
   // One cycle on the calling thread (also what the runner thread calls); returns an NB_* code.
   int RunOneComputation();
-  // Pulls the device state into the host collection (for output or inspection).
+  // Pulls the whole device state into the host collection (for output or inspection).
   int SyncToHost();
+  // nb_body <-> Body for the fields the device holds
+  static nb_body ToDevice(const Body &b);
+  static void FromDevice(const nb_body &d, Body *b);
   RunnerStats stats() const;
   void PrintStats() const;  // computation-runner.go:61-71
 
@@ -206,7 +213,7 @@ class ComputationRunner {
   BodyCollection *bc_;
   nb_ctx *ctx_ = nullptr;
   std::string err_;
-  bool dirty_ = true;       // host collection changed since the last nb_load
+  bool dirty_ = true;       // nothing loaded on the device yet
   bool host_stale_ = false; // device advanced since the last pull
   int max_iteration_ = 0;
   std::atomic<bool> running_{false}, stop_{false};

@@ -19,6 +19,11 @@
 //   --no-collisions         skip detection and resolution
 //   --fp64                  FP64 force path (bit-identical to the reference's arithmetic)
 //   --remove N@K            before cycle K, ask the runner to remove N bodies (RemoveBodies)
+//   --add PATH@K            before cycle K, enqueue every body of the CSV at PATH (AddBody);
+//                           may repeat
+//   --mod ID@K:M1,M2,...    before cycle K, ModBody on body id ID with "name=value" mods;
+//                           may repeat
+//   --get ID@K              before cycle K, print GetBody's answer for body id ID; may repeat
 //   --out PATH              write the surviving bodies as CSV after the run
 //   --parse-only            parse the CSV, print one line per body and exit (no device use)
 #include <chrono>
@@ -37,7 +42,8 @@ static void usage() {
                "[--iterations N]\n"
                "                 [--restitution R] [--G G] [--eps EPS] [--no-cutoff] "
                "[--no-collisions] [--fp64]\n"
-               "                 [--remove N@K] [--out final.csv] [--parse-only]\n");
+               "                 [--remove N@K] [--add PATH@K] [--mod ID@K:M,...] [--get ID@K]\n"
+               "                 [--out final.csv] [--parse-only]\n");
 }
 
 int main(int argc, char **argv) {
@@ -46,6 +52,19 @@ int main(int argc, char **argv) {
   double scaling = 1e-9, R = 1.0;
   int32_t collision = NB_CB_ELASTIC;
   bool parse_only = false;
+  struct Action {
+    int at;
+    int kind;  // 0 add, 1 mod, 2 get
+    std::string path;
+    int64_t id = -1;
+    std::vector<std::string> mods;
+  };
+  std::vector<Action> actions;
+  auto split_at = [](const std::string &s, std::string *head, int *k) {
+    const size_t at = s.find('@');
+    *head = s.substr(0, at);
+    *k = at == std::string::npos ? 0 : std::atoi(s.substr(at + 1).c_str());
+  };
   nb_params p;
   nb_params_init(&p);
   for (int i = 1; i < argc; i++) {
@@ -82,6 +101,35 @@ int main(int argc, char **argv) {
       const size_t at = s.find('@');
       remove_n = std::atoi(s.substr(0, at).c_str());
       remove_at = at == std::string::npos ? 0 : std::atoi(s.substr(at + 1).c_str());
+    } else if (a == "--add") {
+      Action ac;
+      ac.kind = 0;
+      split_at(val(), &ac.path, &ac.at);
+      actions.push_back(ac);
+    } else if (a == "--mod") {
+      const std::string s = val();
+      const size_t colon = s.find(':');
+      Action ac;
+      ac.kind = 1;
+      std::string head;
+      split_at(s.substr(0, colon), &head, &ac.at);
+      ac.id = std::atoll(head.c_str());
+      std::string rest = colon == std::string::npos ? "" : s.substr(colon + 1);
+      size_t q = 0;
+      while (q <= rest.size() && !rest.empty()) {
+        const size_t c = rest.find(',', q);
+        ac.mods.push_back(rest.substr(q, c == std::string::npos ? std::string::npos : c - q));
+        if (c == std::string::npos) break;
+        q = c + 1;
+      }
+      actions.push_back(ac);
+    } else if (a == "--get") {
+      Action ac;
+      ac.kind = 2;
+      std::string head;
+      split_at(val(), &head, &ac.at);
+      ac.id = std::atoll(head.c_str());
+      actions.push_back(ac);
     } else if (a == "--out") out = val();
     else if (a == "--parse-only") parse_only = true;
     else if (a == "-h" || a == "--help") {
@@ -125,6 +173,27 @@ int main(int argc, char **argv) {
         std::chrono::steady_clock::now() - t0 >= std::chrono::milliseconds(run_millis))
       break;
     if (it == remove_at && remove_n != 0) runner.RemoveBodies(remove_n);
+    for (const auto &ac : actions) {
+      if (ac.at != it) continue;
+      if (ac.kind == 0) {
+        std::vector<nbr::Body> add;
+        if (!nbr::FromCsv(ac.path, 1 << 30, collision, 0, &add, &err)) {
+          std::fprintf(stderr, "add: %s\n", err.c_str());
+          return 1;
+        }
+        for (const auto &b : add) runner.AddBody(b);
+      } else if (ac.kind == 1) {
+        std::printf("mod %lld: %d\n", (long long)ac.id, runner.ModBody(ac.id, "", "", ac.mods));
+      } else {
+        const auto b = runner.GetBody(ac.id, "");
+        if (b)
+          std::printf("get %lld: %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %d\n",
+                      (long long)ac.id, b->x, b->y, b->z, b->vx, b->vy, b->vz, b->mass,
+                      b->radius, b->exists ? 1 : 0);
+        else
+          std::printf("get %lld: none\n", (long long)ac.id);
+      }
+    }
     rc = runner.RunOneComputation();
     while (rqh.Next()) {
     }  // the renderer's side: consume the results

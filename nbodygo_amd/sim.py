@@ -122,7 +122,6 @@ class NBodyGPU:
                                 b.beh.ctypes.data_as(L._i32p), b.flags.ctypes.data_as(L._u8p),
                                 _dp(b.rest), _dp(b.frag)))
         self.n = b.n
-        self._template = b
         return self
 
     def step(self, dt, R=1.0, nsteps=1, allow_overflow=False):
@@ -153,12 +152,54 @@ class NBodyGPU:
         return out[:min(k, cnt.value)].copy()
 
     def store(self):
+        """The whole device state as Bodies (nb_store_bodies)."""
         n = self.n
-        b = self._template.copy()
-        self._check(lib.nb_store(self._ctx, _dp(b.x), _dp(b.y), _dp(b.z), _dp(b.vx), _dp(b.vy),
-                                 _dp(b.vz), b.flags.ctypes.data_as(L._u8p), _dp(b.rest)))
-        assert b.n == n
-        return b
+        f = [np.empty(n, np.float64) for _ in range(8)]
+        beh = np.empty(n, np.int32)
+        flags = np.empty(n, np.uint8)
+        rest = np.empty(n, np.float64)
+        frag = np.empty(n, np.float64)
+        self._check(lib.nb_store_bodies(self._ctx, *(_dp(a) for a in f),
+                                        beh.ctypes.data_as(L._i32p),
+                                        flags.ctypes.data_as(L._u8p), _dp(rest), _dp(frag)))
+        return Bodies(*f, beh=beh, flags=flags, rest=rest, frag=frag)
+
+    # -- collection lifecycle on the device (body_collection.go, computation-runner.go) -------
+    def add_bodies(self, b: Bodies):
+        """Queues b for the next cycle() (BodyCollection.Enqueue)."""
+        self._check(lib.nb_add_bodies(self._ctx, b.n, _dp(b.x), _dp(b.y), _dp(b.z), _dp(b.vx),
+                                      _dp(b.vy), _dp(b.vz), _dp(b.m), _dp(b.r),
+                                      b.beh.ctypes.data_as(L._i32p),
+                                      b.flags.ctypes.data_as(L._u8p), _dp(b.frag)))
+
+    def cycle(self, R=1.0):
+        """Compaction of removed bodies, then the queued adds with restitution R
+        (BodyCollection.Cycle).  Returns the new body count."""
+        n = C.c_int64()
+        self._check(lib.nb_cycle(self._ctx, float(R), C.byref(n)))
+        self.n = n.value
+        return self.n
+
+    def remove_bodies(self, count):
+        """processDeletes' removal walk (count, or -1 for all).  Returns the number removed."""
+        k = C.c_int64()
+        self._check(lib.nb_remove_bodies(self._ctx, int(count), C.byref(k)))
+        return k.value
+
+    def get_body(self, i):
+        out = L.NbBody()
+        self._check(lib.nb_get_body(self._ctx, int(i), C.byref(out)))
+        return out.as_dict()
+
+    def set_body(self, i, **fields):
+        """Patches body i (ModBody): the fields given, the rest kept."""
+        cur = L.NbBody()
+        self._check(lib.nb_get_body(self._ctx, int(i), C.byref(cur)))
+        for k, v in fields.items():
+            if k not in cur.as_dict():
+                raise KeyError(k)
+            setattr(cur, k, v)
+        self._check(lib.nb_set_body(self._ctx, int(i), C.byref(cur)))
 
     def render(self):
         xyz = np.empty((self.n, 3), np.float32)

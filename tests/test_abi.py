@@ -39,7 +39,8 @@ def test_struct_layouts_match_header(tmp_path):
     """sizeof/offsetof of every struct field as gcc lays out the header == the ctypes mirror."""
     import subprocess
     structs = {"nb_params": L.NbParams, "nb_stats": L.NbStats, "nb_diag": L.NbDiag,
-               "nb_sur_params": L.NbSurParams, "nb_sur_stats": L.NbSurStats}
+               "nb_sur_params": L.NbSurParams, "nb_sur_stats": L.NbSurStats,
+               "nb_body": L.NbBody}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nbody_gpu.h"',
              "int main(void){"]
     for cname, cls in structs.items():
@@ -105,3 +106,14 @@ def test_surrogate_params_defaults_and_null_handling():
     assert L.lib.nb_sur_get_stats(None, C.byref(st)) == L.NB_E_INVALID
     assert L.lib.nb_sur_save(None, b"x") == L.NB_E_INVALID
     assert L.lib.nb_sur_load(None, b"x") == L.NB_E_INVALID
+
+
+def test_lifecycle_null_handling():
+    b = L.NbBody()
+    n = C.c_int64()
+    assert L.lib.nb_add_bodies(None, 0, *([None] * 11)) == L.NB_E_INVALID
+    assert L.lib.nb_cycle(None, 1.0, C.byref(n)) == L.NB_E_INVALID
+    assert L.lib.nb_remove_bodies(None, 1, C.byref(n)) == L.NB_E_INVALID
+    assert L.lib.nb_get_body(None, 0, C.byref(b)) == L.NB_E_INVALID
+    assert L.lib.nb_set_body(None, 0, C.byref(b)) == L.NB_E_INVALID
+    assert L.lib.nb_store_bodies(None, *([None] * 12)) == L.NB_E_INVALID

@@ -9,6 +9,7 @@ removals and the compaction that follows them: bit for bit without collisions; w
 collisions to 1e-12 relative (coll3d's acos/atan2/sin/cos come from ocml on the device and glibc
 in the oracle, so they may differ in the last ulp).
 """
+import itertools
 import os
 import subprocess
 
@@ -18,6 +19,7 @@ import pytest
 from nbodygo_amd import _lib as L
 from nbodygo_amd import csvio, gen
 from nbodygo_amd import runner as R
+from oracle import lifecycle as LC
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SERVER = os.path.join(ROOT, "nbodygo_amd", "nb_server")
@@ -74,55 +76,50 @@ def test_cpp_csv_loader_stops_before_a_later_malformed_record(tmp_path):
     assert rc == 1 and "quoted-field" in err
 
 
-def test_delete_rule_mirrors_the_reference_walk():
-    ex = np.ones(10, bool)
-    pin = np.zeros(10, bool)
-    # step = 10 // 3 = 3: stride points 0, 3, 6, 9; the fourth removal is the closure quirk
-    assert R.remove_indices(ex, pin, 3) == [0, 3, 6, 9]
-    pin[3] = True  # a pinned body passes the removal on to the next live body
-    assert R.remove_indices(ex, pin, 3) == [0, 4, 6, 9]
-    ex[0] = False
-    assert R.remove_indices(ex, pin, 3) == [1, 4, 6, 9]
-    assert R.remove_indices(ex, pin, 50) == [1, 2, 4, 5, 6, 7, 8, 9]  # more than exist: step 1
-    assert R.remove_indices(ex, pin, -1) == [1, 2, 3, 4, 5, 6, 7, 8, 9]
-    assert R.remove_indices(ex, pin, 0) == []
-
-
-def test_collection_cycle_compacts_and_appends():
-    b = gen.uniform_sphere(6, seed=1)
-    bc = R.BodyCollection(b)
-    b.flags[[1, 4]] = 0
-    bc.b.flags[[1, 4]] = 0
-    ids = bc.ids.copy()
-    add = gen.uniform_sphere(2, seed=2)
-    bc.enqueue_add(add)
-    assert bc.cycle(0.5)
-    assert bc.count() == 6 and list(bc.ids[:4]) == list(ids[[0, 2, 3, 5]])
-    assert np.all(bc.b.rest[4:] == 0.5)
-    assert not bc.cycle(0.5)
-
-
 # ---------------------------------------------------------------------------------------------
 # GPU: both drivers against the oracle
 # ---------------------------------------------------------------------------------------------
-def oracle_run(oracle, b, steps, dt, removals=(), collisions=True):
-    """Oracle state after `steps` cycles; removals = [(cycle, count)] applied before that cycle
-    with the reference's delete rule on the (uncompacted) array."""
+def oracle_run(oracle, b, steps, dt, removals=(), collisions=True, adds=(), mods=(), gets=()):
+    """The runner's cycles on the oracle: before cycle k the removals [(k, count)] (the delete
+    walk over the compacted array), the mods [(k, id, {field: value})] and gets [(k, id)]; then
+    the step; then Cycle with the adds [(k, Bodies)] enqueued before cycle k.  Ids are 0..n-1
+    for the loaded bodies and count on through each add in enqueue order.  Returns the final
+    live state and the get answers (the body's row before that cycle's step)."""
     st = oracle.State.from_bodies(b)
+    ids = np.arange(b.n)
+    nid = b.n
+    answers = {}
     for k in range(steps):
         for at, cnt in removals:
             if at == k:
-                live = (st.flags & L.NB_F_EXISTS).astype(bool)
-                # the drivers compact after each cycle, so the walk sees live bodies only
-                order = np.nonzero(live)[0]
-                idx = R.remove_indices(np.ones(order.size, bool),
-                                       (st.flags[order] & L.NB_F_PINNED).astype(bool), cnt)
-                for i in order[idx]:
-                    st.flags[i] &= ~np.uint8(L.NB_F_EXISTS)
-                    st.a["m"][i] = 0.0
+                idx = LC.remove_indices((st.flags & L.NB_F_EXISTS).astype(bool),
+                                        (st.flags & L.NB_F_PINNED).astype(bool), cnt)
+                LC.apply_removals(st, idx)
+        for at, id_, fields in mods:
+            if at == k:
+                for i in np.nonzero(ids == id_)[0]:
+                    for f, v in fields.items():
+                        getattr(st, f)[i] = v
+        for at, id_ in gets:
+            if at == k:
+                i = np.nonzero(ids == id_)[0][0]
+                answers[(k, id_)] = [getattr(st, f)[i] for f in
+                                     ("x", "y", "z", "vx", "vy", "vz", "m", "r")]
         oracle.step(st, G_REF, 0.0, True, collisions, dt, 1.0, 1, nthreads=16)
+        pend = [a for at, a in adds if at == k]
+        keep = (st.flags & L.NB_F_EXISTS).astype(bool)
+        if pend or not keep.all():
+            new_ids = []
+            for a in pend:
+                new_ids.append(np.arange(nid, nid + a.n))
+                nid += a.n
+            ids = np.concatenate([ids[keep]] + [x[::-1] for x in new_ids[::-1]])
+            out = LC.cycle(st, pend, 1.0)
+            st = oracle.State(*(out[f] for f in ("x", "y", "z", "vx", "vy", "vz", "m", "r")),
+                              beh=out["beh"], flags=out["flags"], rest=out["rest"],
+                              frag=out["frag"])
     live = (st.flags & L.NB_F_EXISTS).astype(bool)
-    return np.stack([st.x, st.y, st.z, st.vx, st.vy, st.vz], 1)[live]
+    return np.stack([st.x, st.y, st.z, st.vx, st.vy, st.vz], 1)[live], answers
 
 
 def assert_tracks(got, exp, collisions):
@@ -151,8 +148,45 @@ def test_cpp_driver_matches_oracle_fp64(oracle, tmp_path, collisions, removals):
     assert res.returncode == 0, res.stderr
     assert "computations: 40" in res.stdout
     got, _, _ = csvio.read_csv(str(out), body_count=10 ** 6)
-    exp = oracle_run(oracle, b, 40, 1e-3, removals, collisions)
+    exp, _ = oracle_run(oracle, b, 40, 1e-3, removals, collisions)
     assert_tracks(np.stack([got.x, got.y, got.z, got.vx, got.vy, got.vz], 1), exp, collisions)
+
+
+@pytest.mark.gpu
+def test_cpp_driver_adds_mods_and_gets_match_oracle(oracle, tmp_path):
+    """AddBody, ModBody and GetBody through nb_server: adds land in Cycle newest first, mods
+    patch the device between cycles (EXISTS=false compacts the body away), gets read it."""
+    b = gen.uniform_sphere(1000, seed=14, radius_k=0.05)
+    src = tmp_path / "c1.csv"
+    csvio.write_csv(b, str(src))
+    b, _, _ = csvio.read_csv(str(src))
+    a = gen.uniform_sphere(40, seed=15, radius_k=0.05)
+    a.x += 0.5
+    add = tmp_path / "add.csv"
+    csvio.write_csv(a, str(add))
+    a, _, _ = csvio.read_csv(str(add))
+    out = tmp_path / "final.csv"
+    cmd = [SERVER, "-f", str(src), "-b", "1000", "-m", "1e-3", "--iterations", "30", "--fp64",
+           "--out", str(out), "--add", "%s@5" % add, "--remove", "13@9",
+           "--mod", "3@12:X=0.25,VX=-0.5,MASS=2e3", "--mod", "1020@14:EXISTS=false",
+           "--mod", "999999@14:X=1", "--get", "3@12", "--get", "1005@20", "--get", "7@25"]
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    assert "mod 3: 3" in res.stdout and "mod 1020: 3" in res.stdout
+    assert "mod 999999: 0" in res.stdout
+    got, _, _ = csvio.read_csv(str(out), body_count=10 ** 6)
+    exp, ans = oracle_run(oracle, b, 30, 1e-3, ((9, 13),), True, adds=((5, a),),
+                          mods=((12, 3, dict(x=0.25, vx=-0.5, m=2e3)),
+                                (14, 1020, dict(flags=0))),
+                          gets=((12, 3), (20, 1005), (25, 7)))
+    assert_tracks(np.stack([got.x, got.y, got.z, got.vx, got.vy, got.vz], 1), exp, True)
+    lines = {l.split(":")[0]: l.split(":")[1].split() for l in res.stdout.splitlines()
+             if l.startswith("get ")}
+    for (k, id_), row in ans.items():
+        vals = lines["get %d" % id_]
+        if vals == ["none"]:
+            assert False, (k, id_)
+        np.testing.assert_allclose(np.array(vals[:8], float), row, rtol=1e-12, atol=1e-15)
 
 
 @pytest.mark.gpu
@@ -172,9 +206,54 @@ def test_python_runner_matches_oracle_fp64(oracle, collisions):
         got = np.stack([bc.b.x, bc.b.y, bc.b.z, bc.b.vx, bc.b.vy, bc.b.vz], 1)
     finally:
         run.close()
-    exp = oracle_run(oracle, b, 30, 1e-3, ((5, 9),), collisions)
+    exp, _ = oracle_run(oracle, b, 30, 1e-3, ((5, 9),), collisions)
     assert_tracks(got, exp, collisions)
-    assert run.stats["computations"] == 30 and run.stats["loads"] >= 2
+    assert run.stats["computations"] == 30 and run.stats["loads"] == 1
+    assert run.stats["compactions"] >= 1
+
+
+@pytest.mark.gpu
+def test_python_runner_adds_mods_and_gets_match_oracle(oracle, monkeypatch):
+    b = gen.uniform_sphere(800, seed=16, radius_k=0.05)
+    a1 = gen.uniform_sphere(30, seed=17, radius_k=0.05)
+    a2 = gen.uniform_sphere(20, seed=18, radius_k=0.05)
+    a2.y -= 0.4
+    rqh = R.ResultQueueHolder(4)
+    bc = R.BodyCollection(b, ids=np.arange(800))
+    monkeypatch.setattr(R, "_ids", itertools.count(800))  # ids go on as the oracle counts
+    run = R.ComputationRunner(1, 1e-3, rqh, bc, precision="fp64", collisions=True)
+    got_rows = {}
+    try:
+        for k in range(25):
+            if k == 4:
+                bc.enqueue_add(a1)
+                bc.enqueue_add(a2)
+            if k == 8:
+                run.remove_bodies(11)
+            if k == 10:
+                assert run.mod_body(5, x=0.3, vz=0.2) == 1
+                assert run.mod_body(845, flags=0) == 1
+                assert run.mod_body(10 ** 7, x=1.0) == 0
+            if k in (10, 20):
+                for id_ in (5, 812):
+                    g = run.get_body(id_)
+                    got_rows[(k, id_)] = [getattr(g, f)[0] for f in
+                                          ("x", "y", "z", "vx", "vy", "vz", "m", "r")]
+            n = bc.count()
+            run.run_one_computation()
+            q = rqh.next()
+            assert q is not None and len(q.queue) == n
+        run.sync_to_host()
+        got = np.stack([bc.b.x, bc.b.y, bc.b.z, bc.b.vx, bc.b.vy, bc.b.vz], 1)
+    finally:
+        run.close()
+    exp, ans = oracle_run(oracle, b, 25, 1e-3, ((8, 11),), True, adds=((4, a1), (4, a2)),
+                          mods=((10, 5, dict(x=0.3, vz=0.2)), (10, 845, dict(flags=0))),
+                          gets=((10, 5), (10, 812), (20, 5), (20, 812)))
+    assert_tracks(got, exp, True)
+    for key, row in ans.items():
+        np.testing.assert_allclose(got_rows[key], row, rtol=1e-12, atol=1e-15)
+    assert run.stats["loads"] == 1
 
 
 @pytest.mark.gpu
