@@ -39,6 +39,18 @@ class OrStats(C.Structure):
                 ("removed_total", C.c_int64)]
 
 
+class OrBhParams(C.Structure):
+    _fields_ = [("G", C.c_double), ("theta", C.c_double), ("soft", C.c_double),
+                ("max_depth", C.c_int32), ("pad", C.c_int32)]
+
+
+class OrBhInfo(C.Structure):
+    _fields_ = [("nodes", C.c_int64), ("internal", C.c_int64), ("buckets", C.c_int64),
+                ("interactions", C.c_int64), ("max_depth_seen", C.c_int32), ("pad", C.c_int32),
+                ("root_mass", C.c_double), ("root_half", C.c_double),
+                ("root_cm", C.c_double * 3)]
+
+
 _dp = C.POINTER(C.c_double)
 _lib = None
 
@@ -67,6 +79,9 @@ def lib():
         L.or_bench_workpool.argtypes = [sp, C.c_double, C.c_int64, C.c_int, C.c_int, _dp,
                                         C.POINTER(C.c_int64)]
         L.or_bench_workpool.restype = C.c_double
+        L.or_bh_force.argtypes = [sp, C.POINTER(OrBhParams), _dp, _dp, _dp,
+                                  C.POINTER(C.c_int32), C.POINTER(OrBhInfo)]
+        L.or_bh_force.restype = C.c_int
         _lib = L
     return _lib
 
@@ -200,3 +215,19 @@ def bench_workpool(st, G, ntargets, nsteps=1, nthreads=8, want_force=False):
     el = lib().or_bench_workpool(C.byref(st.s), G, ntargets, nsteps, nthreads,
                                  f.ctypes.data_as(_dp) if want_force else None, C.byref(hits))
     return el, hits.value, f
+
+
+def bh_force(st, G=6.673e-11, theta=0.3, soft=1e-4, max_depth=21):
+    """Barnes-Hut force (or_bh_force): (n x 3 forces, per-body leaf depth, tree info)."""
+    p = OrBhParams(G, theta, soft, max_depth, 0)
+    f = np.zeros((3, st.n))
+    depth = np.zeros(max(st.n, 1), np.int32)
+    info = OrBhInfo()
+    rc = lib().or_bh_force(C.byref(st.s), C.byref(p), *(f[c].ctypes.data_as(_dp)
+                                                        for c in range(3)),
+                           depth.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(info))
+    if rc:
+        raise RuntimeError("or_bh_force failed %d" % rc)
+    d = {k[0]: getattr(info, k[0]) for k in OrBhInfo._fields_ if k[0] != "pad"}
+    d["root_cm"] = np.array(list(info.root_cm))
+    return f.T.copy(), depth[:st.n].copy(), d

@@ -122,6 +122,37 @@ void or_energy(const or_state *s, double G, double eps, double *out, int nthread
 double or_bench_workpool(const or_state *s, double G, int64_t ntargets, int nsteps,
                          int nthreads, double *fx_out, int64_t *hits_out);
 
+/* Barnes-Hut octree of cmd/body/octree.go with Build wired into BuildTree (nb_oracle_bh.c):
+ * the reference's tree and traversal op by op, plus a depth limit (a leaf at max_depth keeps
+ * further bodies as a bucket).  Pinned by: theta = 0 against the softened direct sum (every
+ * leaf visited, long double brute force), hand-built trees of two to four bodies, the root's
+ * mass and centre of mass, one body per leaf below the limit, and the far-field limit of a
+ * distant cluster (monopole closed form). */
+typedef struct or_bh_params {
+  double G;          /* 6.673e-11 in the reference (body.go:19) */
+  double theta;      /* Theta = 0.3 (octree.go:9) */
+  double soft;       /* Softening = 1e-4 (octree.go:14) */
+  int32_t max_depth; /* deepest node that may hold bodies; deeper splits become buckets */
+  int32_t pad;
+} or_bh_params;
+
+typedef struct or_bh_info {
+  int64_t nodes;          /* pool nodes allocated (Octree.used) */
+  int64_t internal;       /* subdivided nodes */
+  int64_t buckets;        /* leaves holding more than one body (depth limit) */
+  int64_t interactions;   /* accepted nodes + direct terms over all targets */
+  int32_t max_depth_seen; /* deepest leaf holding a body */
+  int32_t pad;
+  double root_mass, root_half, root_cm[3];
+} or_bh_info;
+
+/* Compute's force with the tree (body.go:148-157; CalcForce, octree.go:247-272) for every body:
+ * force components, zero for bodies that do not exist or are fragmenting.  depth (may be NULL)
+ * receives each body's leaf depth (-1 if not in the tree).  Returns 0, or -1 on allocation
+ * failure. */
+int or_bh_force(const or_state *s, const or_bh_params *p, double *fx, double *fy, double *fz,
+                int32_t *depth, or_bh_info *info);
+
 #ifdef __cplusplus
 }
 #endif
