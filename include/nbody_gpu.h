@@ -203,6 +203,45 @@ int nb_remove_bodies(nb_ctx *ctx, int64_t count, int64_t *removed);
 int nb_get_body(nb_ctx *ctx, int64_t index, nb_body *out);
 int nb_set_body(nb_ctx *ctx, int64_t index, const nb_body *in);
 
+/* ---- Barnes-Hut octree (the reference's own far-field design) ------------------------------ */
+/* cmd/body/octree.go with Build wired into BuildTree: the reference allocates an empty tree each
+ * cycle and never builds it (body_collection.go:298-300, octree.go:248-250), so its default
+ * --barnes-hut mode computes no gravity.  When enabled on an FP64 context, nb_step and
+ * nb_compute_forces compute Compute's force as Tree.CalcForce (body.go:156-157): the octree of
+ * the existing, non-fragmenting bodies (Build, octree.go:136-194: leaf capacity 1, the
+ * reference's root cell and centre arithmetic), accumForce's walk (octree.go:276-309) for every
+ * body and directForce sums for suns (octree.go:252-267).  G comes from nb_params; eps and
+ * overlap_cutoff are not used (the tree's softening applies).  Collision detection keeps the
+ * reference's exact test (body.go:192-208), pruned by the tree.  Node centres of mass are summed
+ * bottom-up rather than folded in insertion order, so forces agree with the reference's to
+ * rounding (bit for bit at theta = 0). */
+typedef struct nb_bh_params {
+  double theta;      /* opening test (2*half/dist) < theta; Theta = 0.3 (octree.go:9) */
+  double softening;  /* Softening = 1e-4 (octree.go:14) */
+  int32_t max_depth; /* 1..21: a cell at this depth no longer splits; bodies closer than the
+                        root cell / 2^max_depth share it (the reference recurses without end
+                        for coincident bodies).  Default 21. */
+  int32_t reserved[7];
+} nb_bh_params;
+
+typedef struct nb_bh_stats {
+  int64_t builds;       /* trees built (force phases) */
+  int64_t bodies;       /* bodies in the last tree */
+  int64_t nodes;        /* its cells holding two or more bodies (split cells and buckets) */
+  int64_t buckets;      /* cells at max_depth holding two or more bodies */
+  int64_t interactions; /* accepted cells + body-body terms of the last force phase */
+  int32_t depth_max;    /* deepest leaf */
+  int32_t overflow;     /* nonzero: an internal capacity was exceeded (forces not valid) */
+} nb_bh_stats;
+
+int nb_bh_params_init(nb_bh_params *p);
+/* NB_E_UNSUPPORTED on an FP32 context or with the surrogate on.  p may be NULL (defaults). */
+int nb_bh_enable(nb_ctx *ctx, const nb_bh_params *p);
+int nb_bh_disable(nb_ctx *ctx);
+int nb_bh_get_stats(nb_ctx *ctx, nb_bh_stats *out);
+/* Each body's leaf depth in the last tree (root = 0), -1 for bodies outside it. */
+int nb_bh_get_depths(nb_ctx *ctx, int32_t *depth);
+
 /* ---- batched path: many independent simulations in one persistent launch ------------------ */
 /* No counterpart in the reference (one BodyCollection per process); for callers that run many
  * small simulations (parameter sweeps, ensembles).  Each simulation follows the per-cycle

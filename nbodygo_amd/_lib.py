@@ -116,6 +116,20 @@ class NbSurStats(C.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad1"}
 
 
+class NbBhParams(C.Structure):
+    _fields_ = [("theta", C.c_double), ("softening", C.c_double), ("max_depth", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
+
+
+class NbBhStats(C.Structure):
+    _fields_ = [("builds", C.c_int64), ("bodies", C.c_int64), ("nodes", C.c_int64),
+                ("buckets", C.c_int64), ("interactions", C.c_int64), ("depth_max", C.c_int32),
+                ("overflow", C.c_int32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
 class NbBody(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "r", "rest",
                                            "frag")] + [
@@ -161,6 +175,11 @@ SIGNATURES = [
     ("nb_remove_bodies", C.c_int, [_ctx, C.c_int64, _i64p]),
     ("nb_get_body", C.c_int, [_ctx, C.c_int64, C.POINTER(NbBody)]),
     ("nb_set_body", C.c_int, [_ctx, C.c_int64, C.POINTER(NbBody)]),
+    ("nb_bh_params_init", C.c_int, [C.POINTER(NbBhParams)]),
+    ("nb_bh_enable", C.c_int, [_ctx, C.POINTER(NbBhParams)]),
+    ("nb_bh_disable", C.c_int, [_ctx]),
+    ("nb_bh_get_stats", C.c_int, [_ctx, C.POINTER(NbBhStats)]),
+    ("nb_bh_get_depths", C.c_int, [_ctx, _i32p]),
     ("nb_batch_create", C.c_int, [C.POINTER(NbParams), C.POINTER(_ctx)]),
     ("nb_batch_destroy", None, [_ctx]),
     ("nb_batch_last_error", C.c_char_p, [_ctx]),

@@ -174,6 +174,7 @@ ForceArgs make_force_args(Ctx *c) {
 // Force phase of one cycle (force + detection + sorted pair list).
 static int force_phase(Ctx *c, bool timed, int ev_slot) {
   if (c->sur_on) return sur_force_phase(c, timed, ev_slot);
+  if (c->bh_on) return bh_force_phase(c, timed, ev_slot);
   NB_HIP(c, launch_zero_counters(c->ctr, 0, c->stream));
   if (c->p.precision == NB_PREC_FP64) {
     Force64Args a;
@@ -313,6 +314,8 @@ void nb_destroy(nb_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   sur_free(c->sur);
   c->sur = nullptr;
+  bh_free(c->bh);
+  c->bh = nullptr;
   free_all(c);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -1,0 +1,905 @@
+// nb_bh.hip -- the reference's Barnes-Hut octree (cmd/body/octree.go) on the device, with Build
+// wired into BuildTree (the reference only allocates an empty tree, body_collection.go:298-300,
+// so its default mode computes no gravity).  FP64 throughout, with the reference's arithmetic
+// for every interaction.
+//
+// Build, every force phase:
+//   box     min/max over existing, non-fragmenting bodies; the root's centre and half exactly as
+//           Build computes them (octree.go:136-194)
+//   keys    each body descends from the root with the reference's centre arithmetic (octantOf,
+//           childCenter: octree.go:55-88), one octal digit per level, max_depth levels
+//   sort    stable radix sort of the keys: every node's bodies are one contiguous run, and a
+//           bucket's bodies (the depth limit) stay in array order
+//   levels  one launch per level splits each node's run into its eight octant runs by binary
+//           search on that level's digit.  Runs of two or more bodies become nodes, single
+//           bodies leaves -- the tree that leaf-capacity-1 insertion builds (insert, subdivide:
+//           octree.go:198-231): same topology, node centres, halves and leaf depths.
+//   mass    bottom-up per level.  A leaf's mass and centre of mass are updateMass's from zero,
+//           (0 + x*m) / (0 + m) (octree.go:91-97); a node sums its children in octant order.
+//           (The reference folds bodies into each node one at a time in array order, so a
+//           node's centre of mass can differ from it in the last bits; leaves are identical.)
+//           A bucket folds its bodies in array order as updateMass does.
+// Force (accumForce, octree.go:276-309): one lane per body, bodies in key order.  A wave walks
+// the tree depth first with a shared LDS stack of (node, lane mask): each lane accepts or opens a
+// node by its own test (2*half/dist) < theta, children are pushed in reverse octant order, so
+// every body's sum has the reference's terms in the reference's order.  Suns sum directly over
+// all bodies in array order (CalcForce, octree.go:252-267).
+// Collisions: a second walk opens every node whose box lies within R_i + R_max of the body and
+// applies the reference's test at the leaves (Collided, body.go:192-208); pairs go to the same
+// canonical list as the direct path.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "nb_internal.hpp"
+
+namespace nb {
+
+constexpr int kBhLevels = 21;        // octal digits in a 64-bit key
+constexpr int kBhStack = 160;        // per-wave stack entries (bound: 7 * 20 + 8 = 148)
+constexpr int32_t kBucket = INT32_MIN;
+constexpr int kSunChunk = 64;
+
+struct BhCounters {
+  int32_t M;                         // bodies in the tree
+  int32_t depth_max;                 // deepest leaf
+  int32_t level[kBhLevels + 2];      // first node of each level
+  int32_t lcount[kBhLevels + 2];     // nodes at each level
+  uint32_t nsun;
+  uint32_t buckets;
+  uint32_t stack_overflow;
+  uint32_t node_overflow;
+  unsigned long long interactions;   // accepted nodes + leaf and direct terms, all targets
+};
+
+struct Bh {
+  nb_bh_params p{};
+  int64_t cap = 0, node_cap = 0;
+  uint64_t *keys_in = nullptr, *keys = nullptr;
+  int32_t *perm_in = nullptr, *perm = nullptr;
+  double *sx = nullptr, *sy = nullptr, *sz = nullptr, *sm = nullptr;
+  double *lcx = nullptr, *lcy = nullptr, *lcz = nullptr;
+  int32_t *depth = nullptr, *suns = nullptr;
+  double4 *nbox = nullptr, *nmc = nullptr;
+  int32_t *nchild = nullptr;
+  int2 *nrange = nullptr;
+  double *bpart = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+  BhCounters *cnt = nullptr;
+  int64_t builds = 0;
+};
+
+template <class T>
+static hipError_t balloc(T **p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  return hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
+__device__ __forceinline__ bool bh_active(uint8_t f) {
+  return (f & NB_F_EXISTS) && !(f & NB_F_FRAGMENTING);
+}
+
+__device__ __forceinline__ bool bh_collide_b(int32_t b) {
+  return b == NB_CB_ELASTIC || b == NB_CB_FRAGMENT;
+}
+
+__device__ __forceinline__ void bh_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// ---- box ---------------------------------------------------------------------------------------
+// Partial min/max (x, y, z) and count over existing, non-fragmenting bodies: 7 values per block.
+__global__ __launch_bounds__(256) void k_bh_box_part(const double *x, const double *y,
+                                                     const double *z, const uint8_t *flags, int n,
+                                                     double *part) {
+  __shared__ double sh[7][256];
+  double v[7] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (!bh_active(flags[i])) continue;
+    const double px = x[i], py = y[i], pz = z[i];
+    v[0] = px < v[0] ? px : v[0];
+    v[1] = py < v[1] ? py : v[1];
+    v[2] = pz < v[2] ? pz : v[2];
+    v[3] = px > v[3] ? px : v[3];
+    v[4] = py > v[4] ? py : v[4];
+    v[5] = pz > v[5] ? pz : v[5];
+    v[6] += 1.0;
+  }
+  for (int q = 0; q < 7; q++) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const int o = threadIdx.x + s;
+      for (int q = 0; q < 3; q++) {
+        const double a = sh[q][threadIdx.x], b = sh[q][o];
+        sh[q][threadIdx.x] = b < a ? b : a;
+      }
+      for (int q = 3; q < 6; q++) {
+        const double a = sh[q][threadIdx.x], b = sh[q][o];
+        sh[q][threadIdx.x] = b > a ? b : a;
+      }
+      sh[6][threadIdx.x] += sh[6][o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 7) part[blockIdx.x * 8 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// The root cell (Build: cx = (minX + maxX) / 2, half = max extent / 2 * 1.05) and the level
+// bookkeeping.  One thread.
+__global__ void k_bh_root(const double *part, int nparts, BhCounters *cnt, double4 *nbox,
+                          int2 *nrange, int32_t *nchild) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double v[7] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX, 0.0};
+  for (int b = 0; b < nparts; b++) {
+    const double *p = part + b * 8;
+    for (int q = 0; q < 3; q++) v[q] = p[q] < v[q] ? p[q] : v[q];
+    for (int q = 3; q < 6; q++) v[q] = p[q] > v[q] ? p[q] : v[q];
+    v[6] += p[6];
+  }
+  const int M = (int)v[6];
+  cnt->M = M;
+  cnt->depth_max = 0;
+  for (int d = 0; d < kBhLevels + 2; d++) {
+    cnt->level[d] = 0;
+    cnt->lcount[d] = 0;
+  }
+  cnt->level[1] = 1;
+  cnt->lcount[0] = M >= 2 ? 1 : 0;
+  cnt->nsun = 0;
+  cnt->buckets = 0;
+  cnt->stack_overflow = 0;
+  cnt->node_overflow = 0;
+  cnt->interactions = 0;
+  if (M == 0) return;
+  const double ex = v[3] - v[0], ey = v[4] - v[1], ez = v[5] - v[2];
+  const double mx = ey > ez ? ey : ez;  // math.Max on finite values
+  const double half = (ex > mx ? ex : mx) / 2 * 1.05;
+  nbox[0] = make_double4((v[0] + v[3]) / 2, (v[1] + v[4]) / 2, (v[2] + v[5]) / 2, half);
+  nrange[0] = make_int2(0, M);
+  for (int c = 0; c < 8; c++) nchild[c] = -1;
+}
+
+// Octant path of each body from the root, octantOf / childCenter op by op; D digits,
+// left-aligned in 63 bits.  Bodies outside the tree get the largest key.
+__global__ __launch_bounds__(256) void k_bh_keys(const double *x, const double *y,
+                                                 const double *z, const uint8_t *flags, int n,
+                                                 const double4 *nbox, int D, uint64_t *keys,
+                                                 int32_t *perm, int32_t *depth) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  perm[i] = i;
+  depth[i] = -1;
+  if (!bh_active(flags[i])) {
+    keys[i] = ~0ull;
+    return;
+  }
+  const double4 r = nbox[0];
+  double cx = r.x, cy = r.y, cz = r.z, half = r.w;
+  const double px = x[i], py = y[i], pz = z[i];
+  uint64_t key = 0;
+  for (int l = 0; l < D; l++) {
+    const int o = (px >= cx ? 1 : 0) | (py >= cy ? 2 : 0) | (pz >= cz ? 4 : 0);
+    key = (key << 3) | (uint64_t)o;
+    const double q = half / 2;
+    cx = (o & 1) ? cx + q : cx - q;
+    cy = (o & 2) ? cy + q : cy - q;
+    cz = (o & 4) ? cz + q : cz - q;
+    half = half / 2;
+  }
+  keys[i] = key << (3 * (kBhLevels - D));
+}
+
+// Sorted copies of position and mass, and each body's leaf centre of mass (updateMass from
+// zero: (0*0 + x*m) / (0 + m)).
+__global__ __launch_bounds__(256) void k_bh_gather(const int32_t *perm, const double *x,
+                                                   const double *y, const double *z,
+                                                   const double *m, const BhCounters *cnt,
+                                                   double *sx, double *sy, double *sz,
+                                                   double *sm, double *lcx, double *lcy,
+                                                   double *lcz, int32_t *depth) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= cnt->M) return;
+  const int i = perm[s];
+  if (cnt->M == 1) depth[i] = 0;  // the root is the lone body's leaf
+  const double px = x[i], py = y[i], pz = z[i], mm = m[i];
+  const double zero = 0.0;
+  const double tot = zero + mm;
+  sx[s] = px;
+  sy[s] = py;
+  sz[s] = pz;
+  sm[s] = mm;
+  lcx[s] = (zero * zero + px * mm) / tot;
+  lcy[s] = (zero * zero + py * mm) / tot;
+  lcz[s] = (zero * zero + pz * mm) / tot;
+}
+
+// One level of the tree: each node of level d splits its run into octant runs.
+struct BhLevelArgs {
+  const uint64_t *keys;
+  const int32_t *perm;
+  BhCounters *cnt;
+  double4 *nbox;
+  int2 *nrange;
+  int32_t *nchild;
+  int32_t *depth;
+  int64_t node_cap;
+  int d, D;
+};
+
+__global__ __launch_bounds__(256) void k_bh_level(BhLevelArgs a) {
+  BhCounters *cnt = a.cnt;
+  const int d = a.d;
+  const int begin = cnt->level[d], end = begin + cnt->lcount[d];
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt->level[d + 1] = end;
+  const int shift = 60 - 3 * d;
+  const int last = (int)min((int64_t)end, a.node_cap);  // nodes past the capacity were not made
+  for (int k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < last;
+       k += gridDim.x * blockDim.x) {
+    const int2 rg = a.nrange[k];
+    int bnd[9];
+    bnd[0] = rg.x;
+    bnd[8] = rg.y;
+    for (int c = 1; c < 8; c++) {  // first body whose digit is >= c
+      int lo = bnd[c - 1], hi = rg.y;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)((a.keys[mid] >> shift) & 7) < c)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      bnd[c] = lo;
+    }
+    const double4 bx = a.nbox[k];
+    const double q = bx.w / 2;
+    for (int c = 0; c < 8; c++) {
+      const int cn = bnd[c + 1] - bnd[c];
+      int32_t code = -1;
+      if (cn == 1) {
+        code = -2 - bnd[c];
+        a.depth[a.perm[bnd[c]]] = d + 1;
+        atomicMax(&cnt->depth_max, d + 1);
+      } else if (cn >= 2) {
+        const int id = end + atomicAdd(&cnt->lcount[d + 1], 1);
+        if (id >= a.node_cap) {
+          atomicOr(&cnt->node_overflow, 1u);
+        } else {
+          a.nrange[id] = make_int2(bnd[c], bnd[c + 1]);
+          a.nbox[id] = make_double4((c & 1) ? bx.x + q : bx.x - q, (c & 2) ? bx.y + q : bx.y - q,
+                                    (c & 4) ? bx.z + q : bx.z - q, bx.w / 2);
+          int32_t *ch = a.nchild + (size_t)id * 8;
+          if (d + 1 == a.D) {  // depth limit: a bucket of bodies in array order
+            ch[0] = kBucket;
+            ch[1] = bnd[c];
+            ch[2] = cn;
+            for (int q2 = 3; q2 < 8; q2++) ch[q2] = -1;
+            for (int s = bnd[c]; s < bnd[c + 1]; s++) a.depth[a.perm[s]] = d + 1;
+            atomicAdd(&cnt->buckets, 1u);
+            atomicMax(&cnt->depth_max, d + 1);
+          } else {
+            for (int q2 = 0; q2 < 8; q2++) ch[q2] = -1;
+          }
+          code = id;
+        }
+      }
+      a.nchild[(size_t)k * 8 + c] = code;
+    }
+  }
+}
+
+// Mass and centre of mass of the nodes of level d, from their children (or, for a bucket, its
+// bodies folded in array order by updateMass).
+struct BhMassArgs {
+  const BhCounters *cnt;
+  const int32_t *nchild;
+  const int2 *nrange;
+  double4 *nmc;
+  const double *sx, *sy, *sz, *sm, *lcx, *lcy, *lcz;
+  int64_t node_cap;
+  int d;
+};
+
+__global__ __launch_bounds__(256) void k_bh_mass(BhMassArgs a) {
+  const int begin = a.cnt->level[a.d];
+  const int end = (int)min((int64_t)begin + a.cnt->lcount[a.d], a.node_cap);
+  for (int k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end;
+       k += gridDim.x * blockDim.x) {
+    const int32_t *ch = a.nchild + (size_t)k * 8;
+    if (ch[0] == kBucket) {
+      double mt = 0, cx = 0, cy = 0, cz = 0;
+      for (int s = ch[1]; s < ch[1] + ch[2]; s++) {
+        const double mm = a.sm[s];
+        const double nt = mt + mm;
+        cx = (cx * mt + a.sx[s] * mm) / nt;
+        cy = (cy * mt + a.sy[s] * mm) / nt;
+        cz = (cz * mt + a.sz[s] * mm) / nt;
+        mt = nt;
+      }
+      a.nmc[k] = make_double4(cx, cy, cz, mt);
+      continue;
+    }
+    double mt = 0, wx = 0, wy = 0, wz = 0;
+    for (int c = 0; c < 8; c++) {
+      const int32_t code = ch[c];
+      if (code == -1) continue;
+      double mc, px, py, pz;
+      if (code <= -2) {
+        const int s = -2 - code;
+        mc = 0.0 + a.sm[s];
+        px = a.lcx[s];
+        py = a.lcy[s];
+        pz = a.lcz[s];
+      } else {
+        const double4 v = a.nmc[code];
+        mc = v.w;
+        px = v.x;
+        py = v.y;
+        pz = v.z;
+      }
+      mt += mc;
+      wx += px * mc;
+      wy += py * mc;
+      wz += pz * mc;
+    }
+    a.nmc[k] = make_double4(wx / mt, wy / mt, wz / mt, mt);
+  }
+}
+
+// ---- force -------------------------------------------------------------------------------------
+struct BhForceArgs {
+  const BhCounters *cnt;
+  BhCounters *cnt_w;
+  const int32_t *perm;
+  const uint8_t *flags;
+  const double *sx, *sy, *sz, *sm, *lcx, *lcy, *lcz;
+  const double4 *nbox, *nmc;
+  const int32_t *nchild;
+  double *acc;
+  int npad;
+  double G, theta, soft2;
+};
+
+// Mass of a child slot as the reference's child.totalMass (a leaf: 0 + m).
+__device__ __forceinline__ double bh_child_mass(const BhForceArgs &a, int32_t code) {
+  return code <= -2 ? 0.0 + a.sm[-2 - code] : a.nmc[code].w;
+}
+
+__global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
+  __shared__ int32_t stk_code[4][kBhStack];
+  __shared__ unsigned long long stk_mask[4][kBhStack];
+  const int M = a.cnt->M;
+  if (M < 2) return;  // a lone body feels nothing; forces were zeroed
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = -1;
+  bool act = false;
+  double xi = 0, yi = 0, zi = 0, Gm = 0;
+  if (s < M) {
+    i = a.perm[s];
+    act = !(a.flags[i] & NB_F_SUN);  // suns sum directly
+    xi = a.sx[s];
+    yi = a.sy[s];
+    zi = a.sz[s];
+    Gm = a.G * a.sm[s];
+  }
+  const unsigned long long start = __builtin_amdgcn_ballot_w64(act);
+  if (start == 0) return;
+  const double soft2 = a.soft2, theta = a.theta;
+  double fx = 0, fy = 0, fz = 0;
+  unsigned long long nint = 0;
+  int sp = 0;
+  if (a.nmc[0].w != 0.0) {  // accumForce: n.totalMass == 0 returns
+    if (lane == 0) {
+      stk_code[w][0] = 0;
+      stk_mask[w][0] = start;
+    }
+    sp = 1;
+  }
+  bh_wave_sync();
+  bool overflow = false;
+  while (sp > 0) {
+    sp--;
+    const int code = __builtin_amdgcn_readfirstlane(stk_code[w][sp]);
+    const unsigned long long msk = stk_mask[w][sp];
+    const bool on = (msk >> lane) & 1ull;
+    if (code <= -2) {  // a leaf: the body's centre of mass (x*m)/m and mass 0 + m
+      const int s2 = -2 - code;
+      if (on && s2 != s) {
+        const double dx = a.lcx[s2] - xi;
+        const double dy = a.lcy[s2] - yi;
+        const double dz = a.lcz[s2] - zi;
+        const double distSq = dx * dx + dy * dy + dz * dz + soft2;
+        const double dist = __builtin_sqrt(distSq);
+        const double force = Gm * (0.0 + a.sm[s2]) / distSq;
+        fx += force * dx / dist;
+        fy += force * dy / dist;
+        fz += force * dz / dist;
+        nint++;
+      }
+      continue;
+    }
+    const int32_t *ch = a.nchild + (size_t)code * 8;
+    const int32_t c0 = ch[0];
+    if (c0 == kBucket) {  // depth limit: the bucket's bodies one by one (directForce)
+      const int b0 = ch[1], bn = ch[2];
+      if (on)
+        for (int s2 = b0; s2 < b0 + bn; s2++) {
+          if (s2 == s) continue;
+          const double dx = a.sx[s2] - xi;
+          const double dy = a.sy[s2] - yi;
+          const double dz = a.sz[s2] - zi;
+          const double distSq = dx * dx + dy * dy + dz * dz + soft2;
+          const double dist = __builtin_sqrt(distSq);
+          const double force = Gm * a.sm[s2] / distSq;
+          fx += force * dx / dist;
+          fy += force * dy / dist;
+          fz += force * dz / dist;
+          nint++;
+        }
+      continue;
+    }
+    const double4 mc = a.nmc[code];
+    const double half = a.nbox[code].w;
+    bool open = false;
+    if (on) {
+      const double dx = mc.x - xi;
+      const double dy = mc.y - yi;
+      const double dz = mc.z - zi;
+      const double distSq = dx * dx + dy * dy + dz * dz + soft2;
+      const double dist = __builtin_sqrt(distSq);
+      if ((2 * half / dist) < theta) {
+        const double force = Gm * mc.w / distSq;
+        fx += force * dx / dist;
+        fy += force * dy / dist;
+        fz += force * dz / dist;
+        nint++;
+      } else {
+        open = true;
+      }
+    }
+    const unsigned long long om = __builtin_amdgcn_ballot_w64(open);
+    if (om) {
+      if (sp + 8 > kBhStack) {
+        overflow = true;
+      } else {
+        for (int c = 7; c >= 0; c--) {  // children 0..7 come off the stack in order
+          const int32_t cc = ch[c];
+          if (cc == -1 || !(bh_child_mass(a, cc) > 0)) continue;
+          if (lane == 0) {
+            stk_code[w][sp] = cc;
+            stk_mask[w][sp] = om;
+          }
+          sp++;
+        }
+      }
+    }
+    bh_wave_sync();
+  }
+  if (act) {
+    a.acc[i] = fx;
+    a.acc[a.npad + i] = fy;
+    a.acc[2 * a.npad + i] = fz;
+  }
+  for (int o = 32; o > 0; o >>= 1) nint += __shfl_xor(nint, o);
+  if (lane == 0) {
+    atomicAdd(&a.cnt_w->interactions, nint);
+    if (overflow) atomicOr(&a.cnt_w->stack_overflow, 1u);
+  }
+}
+
+// Suns: directForce from every other existing, non-fragmenting body in array order.
+__global__ __launch_bounds__(256) void k_bh_sun_list(const uint8_t *flags, int n, BhCounters *cnt,
+                                                     int32_t *suns) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && bh_active(flags[i]) && (flags[i] & NB_F_SUN)) suns[atomicAdd(&cnt->nsun, 1u)] = i;
+}
+
+struct BhSunArgs {
+  BhCounters *cnt;
+  const int32_t *suns;
+  const double *x, *y, *z, *m;
+  const uint8_t *flags;
+  double *acc;
+  int n, npad;
+  double G, soft2;
+};
+
+__global__ __launch_bounds__(64) void k_bh_sun(BhSunArgs a) {
+  __shared__ double t[3][kSunChunk];
+  __shared__ int valid[kSunChunk];
+  if (a.cnt->M < 2) return;
+  const int lane = threadIdx.x;
+  const int nsun = (int)a.cnt->nsun;
+  for (int k = blockIdx.x; k < nsun; k += gridDim.x) {
+    const int b = a.suns[k];
+    const double xb = a.x[b], yb = a.y[b], zb = a.z[b], Gm = a.G * a.m[b];
+    double sum = 0;  // lane c < 3 accumulates component c
+    unsigned long long nint = 0;
+    for (int j0 = 0; j0 < a.n; j0 += kSunChunk) {
+      const int j = j0 + lane;
+      const bool v = j < a.n && j != b && bh_active(a.flags[j]);
+      double tx = 0, ty = 0, tz = 0;
+      if (v) {
+        const double dx = a.x[j] - xb;
+        const double dy = a.y[j] - yb;
+        const double dz = a.z[j] - zb;
+        const double distSq = dx * dx + dy * dy + dz * dz + a.soft2;
+        const double dist = __builtin_sqrt(distSq);
+        const double force = Gm * a.m[j] / distSq;
+        tx = force * dx / dist;
+        ty = force * dy / dist;
+        tz = force * dz / dist;
+        nint++;
+      }
+      t[0][lane] = tx;
+      t[1][lane] = ty;
+      t[2][lane] = tz;
+      valid[lane] = v ? 1 : 0;
+      bh_wave_sync();
+      if (lane < 3)
+        for (int q = 0; q < kSunChunk; q++)
+          if (valid[q]) sum += t[lane][q];
+      bh_wave_sync();
+    }
+    if (lane < 3) a.acc[(size_t)lane * a.npad + b] = sum;
+    for (int o = 32; o > 0; o >>= 1) nint += __shfl_xor(nint, o);
+    if (lane == 0) atomicAdd(&a.cnt->interactions, nint);
+  }
+}
+
+// ---- collision pairs ---------------------------------------------------------------------------
+struct BhDetectArgs {
+  const BhCounters *cnt;
+  const int32_t *perm;
+  const double *x, *y, *z, *r;
+  const int32_t *beh;
+  const double4 *nbox;
+  const int32_t *nchild;
+  uint64_t *pairs;
+  DevCounters *ctr;
+  int64_t pair_cap;
+  double rmax;
+};
+
+__device__ __forceinline__ void bh_emit(const BhDetectArgs &a, int i, int j) {
+  const uint32_t q = atomicAdd(&a.ctr->pair_count, 1u);
+  if ((int64_t)q < a.pair_cap)
+    a.pairs[q] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+  else
+    atomicOr(&a.ctr->pair_overflow, 1u);
+}
+
+// Collided (body.go:192-208) from body i's side, pairs i < j with both behaviours colliding.
+__device__ __forceinline__ void bh_test_pair(const BhDetectArgs &a, int i, double xi, double yi,
+                                             double zi, double ri, int s2) {
+  const int j = a.perm[s2];
+  if (j <= i || !bh_collide_b(a.beh[j])) return;
+  const double dx = a.x[j] - xi;
+  const double dy = a.y[j] - yi;
+  const double dz = a.z[j] - zi;
+  const double dist = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+  if (dist <= ri + a.r[j]) bh_emit(a, i, j);
+}
+
+__global__ __launch_bounds__(256) void k_bh_detect(BhDetectArgs a) {
+  __shared__ int32_t stk_code[4][kBhStack];
+  __shared__ unsigned long long stk_mask[4][kBhStack];
+  const int M = a.cnt->M;
+  if (M < 2) return;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = -1;
+  bool act = false;
+  double xi = 0, yi = 0, zi = 0, ri = 0, lim = 0;
+  if (s < M) {
+    i = a.perm[s];
+    act = bh_collide_b(a.beh[i]);
+    xi = a.x[i];
+    yi = a.y[i];
+    zi = a.z[i];
+    ri = a.r[i];
+    // a box farther than R_i + R_max holds no partner; the slack covers the box test's rounding
+    const double reach = ri + a.rmax;
+    lim = reach + 1e-12 * (fabs(xi) + fabs(yi) + fabs(zi) + reach) + 1e-300;
+  }
+  const unsigned long long start = __builtin_amdgcn_ballot_w64(act);
+  if (start == 0) return;
+  if (lane == 0) {
+    stk_code[w][0] = 0;
+    stk_mask[w][0] = start;
+  }
+  int sp = 1;
+  bh_wave_sync();
+  while (sp > 0) {
+    sp--;
+    const int code = __builtin_amdgcn_readfirstlane(stk_code[w][sp]);
+    const unsigned long long msk = stk_mask[w][sp];
+    const bool on = (msk >> lane) & 1ull;
+    if (code <= -2) {
+      if (on) bh_test_pair(a, i, xi, yi, zi, ri, -2 - code);
+      continue;
+    }
+    const int32_t *ch = a.nchild + (size_t)code * 8;
+    if (ch[0] == kBucket) {
+      if (on)
+        for (int s2 = ch[1]; s2 < ch[1] + ch[2]; s2++)
+          if (s2 != s) bh_test_pair(a, i, xi, yi, zi, ri, s2);
+      continue;
+    }
+    const double4 bx = a.nbox[code];
+    bool pass = false;
+    if (on) {
+      const double ex = fmax(fabs(xi - bx.x) - bx.w, 0.0);
+      const double ey = fmax(fabs(yi - bx.y) - bx.w, 0.0);
+      const double ez = fmax(fabs(zi - bx.z) - bx.w, 0.0);
+      const double slack = 1e-12 * (fabs(bx.x) + fabs(bx.y) + fabs(bx.z) + bx.w);
+      const double l = lim + slack;
+      pass = ex * ex + ey * ey + ez * ez <= l * l;
+    }
+    const unsigned long long pm = __builtin_amdgcn_ballot_w64(pass);
+    if (pm && sp + 8 <= kBhStack) {
+      for (int c = 7; c >= 0; c--) {
+        const int32_t cc = ch[c];
+        if (cc == -1) continue;
+        if (lane == 0) {
+          stk_code[w][sp] = cc;
+          stk_mask[w][sp] = pm;
+        }
+        sp++;
+      }
+    }
+    bh_wave_sync();
+  }
+}
+
+__global__ void k_bh_zero(double *acc, size_t len) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < len;
+       k += (size_t)gridDim.x * blockDim.x)
+    acc[k] = 0.0;
+}
+
+// ---- host --------------------------------------------------------------------------------------
+static int nb256(int64_t n) { return (int)std::max<int64_t>(1, (n + 255) / 256); }
+
+void bh_free(Bh *b) {
+  if (!b) return;
+  void *ps[] = {b->keys_in, b->keys, b->perm_in, b->perm, b->sx,     b->sy,     b->sz,
+                b->sm,      b->lcx,  b->lcy,     b->lcz,  b->depth,  b->suns,   b->nbox,
+                b->nmc,     b->nchild, b->nrange, b->bpart, b->sort_tmp, b->cnt};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+static int bh_ensure(Ctx *c) {
+  Bh *b = c->bh;
+  if (!b->cnt) {
+    NB_HIP(c, hipMalloc((void **)&b->cnt, sizeof(BhCounters)));
+    NB_HIP(c, hipMemset(b->cnt, 0, sizeof(BhCounters)));
+    NB_HIP(c, balloc(&b->bpart, 1024 * 8));
+  }
+  if (b->cap >= c->cap) return NB_OK;
+  const size_t n = (size_t)c->cap;
+  NB_HIP(c, balloc(&b->keys_in, n));
+  NB_HIP(c, balloc(&b->keys, n));
+  NB_HIP(c, balloc(&b->perm_in, n));
+  NB_HIP(c, balloc(&b->perm, n));
+  NB_HIP(c, balloc(&b->sx, n));
+  NB_HIP(c, balloc(&b->sy, n));
+  NB_HIP(c, balloc(&b->sz, n));
+  NB_HIP(c, balloc(&b->sm, n));
+  NB_HIP(c, balloc(&b->lcx, n));
+  NB_HIP(c, balloc(&b->lcy, n));
+  NB_HIP(c, balloc(&b->lcz, n));
+  NB_HIP(c, balloc(&b->depth, n));
+  NB_HIP(c, balloc(&b->suns, n));
+  // every node holds 2+ bodies and the nodes of one level are disjoint: at most n/2 per level
+  b->node_cap = (int64_t)(n / 2) * kBhLevels + 64;
+  NB_HIP(c, balloc(&b->nbox, (size_t)b->node_cap));
+  NB_HIP(c, balloc(&b->nmc, (size_t)b->node_cap));
+  NB_HIP(c, balloc(&b->nchild, (size_t)b->node_cap * 8));
+  NB_HIP(c, balloc(&b->nrange, (size_t)b->node_cap));
+  size_t bytes = 0;
+  NB_HIP(c, rocprim::radix_sort_pairs(nullptr, bytes, b->keys_in, b->keys, b->perm_in, b->perm,
+                                      n, 0, 64, c->stream));
+  void *tmp = nullptr;
+  NB_HIP(c, hipMalloc(&tmp, bytes ? bytes : 1));
+  if (b->sort_tmp) (void)hipFree(b->sort_tmp);
+  b->sort_tmp = tmp;
+  b->sort_bytes = bytes;
+  b->cap = c->cap;
+  return NB_OK;
+}
+
+int bh_force_phase(Ctx *c, bool timed, int ev_slot) {
+  Bh *b = c->bh;
+  hipStream_t st = c->stream;
+  NB_HIP(c, launch_zero_counters(c->ctr, 0, st));
+  int rc = bh_ensure(c);
+  if (rc) return rc;
+  const int n = (int)c->n;
+  const int D = b->p.max_depth;
+  const size_t acc_len = (size_t)3 * c->npad;
+  hipLaunchKernelGGL(k_bh_zero, dim3(nb256((int64_t)std::min<size_t>(acc_len, 1 << 20))),
+                     dim3(256), 0, st, c->acc, acc_len);
+  if (n == 0) return NB_OK;
+  const int nparts = std::min(1024, nb256(n));
+  hipLaunchKernelGGL(k_bh_box_part, dim3(nparts), dim3(256), 0, st, c->x, c->y, c->z, c->flags,
+                     n, b->bpart);
+  hipLaunchKernelGGL(k_bh_root, dim3(1), dim3(64), 0, st, b->bpart, nparts, b->cnt, b->nbox,
+                     b->nrange, b->nchild);
+  hipLaunchKernelGGL(k_bh_keys, dim3(nb256(n)), dim3(256), 0, st, c->x, c->y, c->z, c->flags, n,
+                     b->nbox, D, b->keys_in, b->perm_in, b->depth);
+  NB_HIP(c, hipGetLastError());
+  size_t bytes = b->sort_bytes;
+  NB_HIP(c, rocprim::radix_sort_pairs(b->sort_tmp, bytes, b->keys_in, b->keys, b->perm_in,
+                                      b->perm, (size_t)n, 0, 64, st));
+  hipLaunchKernelGGL(k_bh_gather, dim3(nb256(n)), dim3(256), 0, st, b->perm, c->x, c->y, c->z,
+                     c->m, b->cnt, b->sx, b->sy, b->sz, b->sm, b->lcx, b->lcy, b->lcz, b->depth);
+  const int gl = std::min(2048, nb256(std::max(1, n / 2)));
+  BhLevelArgs la;
+  la.keys = b->keys;
+  la.perm = b->perm;
+  la.cnt = b->cnt;
+  la.nbox = b->nbox;
+  la.nrange = b->nrange;
+  la.nchild = b->nchild;
+  la.depth = b->depth;
+  la.node_cap = b->node_cap;
+  la.D = D;
+  for (int d = 0; d < D; d++) {
+    la.d = d;
+    hipLaunchKernelGGL(k_bh_level, dim3(gl), dim3(256), 0, st, la);
+  }
+  BhMassArgs ma;
+  ma.cnt = b->cnt;
+  ma.nchild = b->nchild;
+  ma.nrange = b->nrange;
+  ma.nmc = b->nmc;
+  ma.sx = b->sx; ma.sy = b->sy; ma.sz = b->sz; ma.sm = b->sm;
+  ma.lcx = b->lcx; ma.lcy = b->lcy; ma.lcz = b->lcz;
+  ma.node_cap = b->node_cap;
+  for (int d = D; d >= 0; d--) {
+    ma.d = d;
+    hipLaunchKernelGGL(k_bh_mass, dim3(gl), dim3(256), 0, st, ma);
+  }
+  NB_HIP(c, hipGetLastError());
+  BhForceArgs fa;
+  fa.cnt = b->cnt;
+  fa.cnt_w = b->cnt;
+  fa.perm = b->perm;
+  fa.flags = c->flags;
+  fa.sx = b->sx; fa.sy = b->sy; fa.sz = b->sz; fa.sm = b->sm;
+  fa.lcx = b->lcx; fa.lcy = b->lcy; fa.lcz = b->lcz;
+  fa.nbox = b->nbox;
+  fa.nmc = b->nmc;
+  fa.nchild = b->nchild;
+  fa.acc = c->acc;
+  fa.npad = (int)c->npad;
+  fa.G = c->p.G;
+  fa.theta = b->p.theta;
+  fa.soft2 = b->p.softening * b->p.softening;
+  if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot], st));
+  hipLaunchKernelGGL(k_bh_force, dim3(nb256(n)), dim3(256), 0, st, fa);
+  hipLaunchKernelGGL(k_bh_sun_list, dim3(nb256(n)), dim3(256), 0, st, c->flags, n, b->cnt,
+                     b->suns);
+  BhSunArgs sa;
+  sa.cnt = b->cnt;
+  sa.suns = b->suns;
+  sa.x = c->x; sa.y = c->y; sa.z = c->z; sa.m = c->m;
+  sa.flags = c->flags;
+  sa.acc = c->acc;
+  sa.n = n;
+  sa.npad = (int)c->npad;
+  sa.G = c->p.G;
+  sa.soft2 = fa.soft2;
+  hipLaunchKernelGGL(k_bh_sun, dim3(64), dim3(64), 0, st, sa);
+  if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], st));
+  NB_HIP(c, hipGetLastError());
+  if (c->p.collisions) {
+    BhDetectArgs da;
+    da.cnt = b->cnt;
+    da.perm = b->perm;
+    da.x = c->x; da.y = c->y; da.z = c->z; da.r = c->r;
+    da.beh = c->beh;
+    da.nbox = b->nbox;
+    da.nchild = b->nchild;
+    da.pairs = c->pairs;
+    da.ctr = c->ctr;
+    da.pair_cap = c->pair_cap;
+    da.rmax = c->rmax;
+    hipLaunchKernelGGL(k_bh_detect, dim3(nb256(n)), dim3(256), 0, st, da);
+    NB_HIP(c, hipGetLastError());
+    NB_HIP(c, launch_sort_pairs(c->pairs, c->pair_cap, c->ctr, st));
+  }
+  b->builds++;
+  return NB_OK;
+}
+
+}  // namespace nb
+
+using namespace nb;
+
+extern "C" {
+
+int nb_bh_params_init(nb_bh_params *p) {
+  if (!p) return NB_E_INVALID;
+  memset(p, 0, sizeof *p);
+  p->theta = 0.3;
+  p->softening = 1e-4;
+  p->max_depth = kBhLevels;
+  return NB_OK;
+}
+
+int nb_bh_enable(nb_ctx *c, const nb_bh_params *p) {
+  if (!c) return NB_E_INVALID;
+  nb_bh_params q;
+  if (p)
+    q = *p;
+  else
+    nb_bh_params_init(&q);
+  if (!(q.theta >= 0) || !std::isfinite(q.theta) || !(q.softening >= 0) ||
+      !std::isfinite(q.softening) || q.max_depth < 1 || q.max_depth > kBhLevels)
+    return fail_msg(c, NB_E_INVALID, "nb_bh_enable: theta, softening >= 0 and max_depth in [1, 21]");
+  if (c->p.precision != NB_PREC_FP64)
+    return fail_msg(c, NB_E_UNSUPPORTED, "nb_bh_enable: the octree path is FP64 (precision fp64)");
+  if (c->sur_on)
+    return fail_msg(c, NB_E_UNSUPPORTED, "nb_bh_enable: the surrogate is on; disable it first");
+  NB_HIP(c, hipSetDevice(c->device));
+  if (!c->bh) c->bh = new Bh();
+  c->bh->p = q;
+  c->bh_on = true;
+  return NB_OK;
+}
+
+int nb_bh_disable(nb_ctx *c) {
+  if (!c) return NB_E_INVALID;
+  c->bh_on = false;
+  return NB_OK;
+}
+
+int nb_bh_get_stats(nb_ctx *c, nb_bh_stats *out) {
+  if (!c || !out) return NB_E_INVALID;
+  memset(out, 0, sizeof *out);
+  if (!c->bh || !c->bh->cnt) return NB_OK;
+  NB_HIP(c, hipSetDevice(c->device));
+  BhCounters h;
+  NB_HIP(c, hipStreamSynchronize(c->stream));
+  NB_HIP(c, hipMemcpy(&h, c->bh->cnt, sizeof h, hipMemcpyDeviceToHost));
+  int64_t nodes = 0;
+  for (int d = 0; d <= kBhLevels; d++) nodes += h.lcount[d];
+  out->builds = c->bh->builds;
+  out->bodies = h.M;
+  out->nodes = nodes;
+  out->buckets = h.buckets;
+  out->interactions = (int64_t)h.interactions;
+  out->depth_max = h.depth_max;
+  out->overflow = (int32_t)(h.stack_overflow | h.node_overflow);
+  return NB_OK;
+}
+
+int nb_bh_get_depths(nb_ctx *c, int32_t *depth) {
+  if (!c || !depth) return NB_E_INVALID;
+  if (!c->bh || !c->bh->depth || c->bh->builds == 0)
+    return fail_msg(c, NB_E_STATE, "nb_bh_get_depths: no tree built yet");
+  if (c->n == 0) return NB_OK;
+  NB_HIP(c, hipSetDevice(c->device));
+  NB_HIP(c, hipMemcpyAsync(depth, c->bh->depth, (size_t)c->n * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
+  return NB_OK;
+}
+
+}  // extern "C"

@@ -35,6 +35,7 @@ struct DevCounters {
 };
 
 struct Sur;  // far-field surrogate state (nb_sur.hip)
+struct Bh;   // Barnes-Hut octree state (nb_bh.hip)
 
 // FP32 force kernel launch shape.
 struct ForceShape {
@@ -92,6 +93,8 @@ struct Ctx {
 
   Sur *sur = nullptr;
   bool sur_on = false;
+  Bh *bh = nullptr;
+  bool bh_on = false;
 
   // bodies queued by nb_add_bodies, appended by the next nb_cycle
   struct Adds {
@@ -150,6 +153,8 @@ ForceArgs make_force_args(Ctx *c);
 int fail_msg(Ctx *c, int code, const char *msg);
 void sur_free(Sur *s);
 int sur_force_phase(Ctx *c, bool timed, int ev_slot);
+void bh_free(Bh *b);
+int bh_force_phase(Ctx *c, bool timed, int ev_slot);
 
 struct FixupArgs {
   const double *x, *y, *z, *m, *r;

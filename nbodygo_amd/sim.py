@@ -164,6 +164,26 @@ class NBodyGPU:
                                         flags.ctypes.data_as(L._u8p), _dp(rest), _dp(frag)))
         return Bodies(*f, beh=beh, flags=flags, rest=rest, frag=frag)
 
+    # -- Barnes-Hut octree (cmd/body/octree.go with Build wired in) ---------------------------
+    def bh_enable(self, theta=0.3, softening=1e-4, max_depth=21):
+        p = L.NbBhParams()
+        check(lib.nb_bh_params_init(C.byref(p)))
+        p.theta, p.softening, p.max_depth = float(theta), float(softening), int(max_depth)
+        self._check(lib.nb_bh_enable(self._ctx, C.byref(p)))
+
+    def bh_disable(self):
+        self._check(lib.nb_bh_disable(self._ctx))
+
+    def bh_stats(self):
+        st = L.NbBhStats()
+        self._check(lib.nb_bh_get_stats(self._ctx, C.byref(st)))
+        return st.as_dict()
+
+    def bh_depths(self):
+        d = np.empty(max(self.n, 1), np.int32)
+        self._check(lib.nb_bh_get_depths(self._ctx, d.ctypes.data_as(L._i32p)))
+        return d[:self.n].copy()
+
     # -- collection lifecycle on the device (body_collection.go, computation-runner.go) -------
     def add_bodies(self, b: Bodies):
         """Queues b for the next cycle() (BodyCollection.Enqueue)."""

@@ -40,7 +40,7 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     structs = {"nb_params": L.NbParams, "nb_stats": L.NbStats, "nb_diag": L.NbDiag,
                "nb_sur_params": L.NbSurParams, "nb_sur_stats": L.NbSurStats,
-               "nb_body": L.NbBody}
+               "nb_body": L.NbBody, "nb_bh_params": L.NbBhParams, "nb_bh_stats": L.NbBhStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nbody_gpu.h"',
              "int main(void){"]
     for cname, cls in structs.items():
@@ -117,3 +117,15 @@ def test_lifecycle_null_handling():
     assert L.lib.nb_get_body(None, 0, C.byref(b)) == L.NB_E_INVALID
     assert L.lib.nb_set_body(None, 0, C.byref(b)) == L.NB_E_INVALID
     assert L.lib.nb_store_bodies(None, *([None] * 12)) == L.NB_E_INVALID
+
+
+def test_barnes_hut_params_defaults_and_null_handling():
+    p = L.NbBhParams()
+    assert L.lib.nb_bh_params_init(C.byref(p)) == L.NB_OK
+    assert (p.theta, p.softening, p.max_depth) == (0.3, 1e-4, 21)  # octree.go:9, :14
+    assert L.lib.nb_bh_params_init(None) == L.NB_E_INVALID
+    assert L.lib.nb_bh_enable(None, C.byref(p)) == L.NB_E_INVALID
+    assert L.lib.nb_bh_disable(None) == L.NB_E_INVALID
+    st = L.NbBhStats()
+    assert L.lib.nb_bh_get_stats(None, C.byref(st)) == L.NB_E_INVALID
+    assert L.lib.nb_bh_get_depths(None, None) == L.NB_E_INVALID
