@@ -221,7 +221,8 @@ typedef struct nb_bh_params {
   int32_t max_depth; /* 1..21: a cell at this depth no longer splits; bodies closer than the
                         root cell / 2^max_depth share it (the reference recurses without end
                         for coincident bodies).  Default 21. */
-  int32_t reserved[7];
+  int32_t group;     /* bodies per wavefront in the tree walk: 16, 32 or 64; 0 = 64 */
+  int32_t reserved[6];
 } nb_bh_params;
 
 typedef struct nb_bh_stats {

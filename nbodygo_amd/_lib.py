@@ -118,7 +118,7 @@ class NbSurStats(C.Structure):
 
 class NbBhParams(C.Structure):
     _fields_ = [("theta", C.c_double), ("softening", C.c_double), ("max_depth", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("group", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class NbBhStats(C.Structure):

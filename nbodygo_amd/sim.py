@@ -165,10 +165,11 @@ class NBodyGPU:
         return Bodies(*f, beh=beh, flags=flags, rest=rest, frag=frag)
 
     # -- Barnes-Hut octree (cmd/body/octree.go with Build wired in) ---------------------------
-    def bh_enable(self, theta=0.3, softening=1e-4, max_depth=21):
+    def bh_enable(self, theta=0.3, softening=1e-4, max_depth=21, group=0):
         p = L.NbBhParams()
         check(lib.nb_bh_params_init(C.byref(p)))
         p.theta, p.softening, p.max_depth = float(theta), float(softening), int(max_depth)
+        p.group = int(group)
         self._check(lib.nb_bh_enable(self._ctx, C.byref(p)))
 
     def bh_disable(self):
