@@ -45,6 +45,10 @@ WORKLOADS = {
                                           "surrogate far field + exact near field (detection "
                                           "there) against the plain FP32 direct sum; energy and "
                                           "momentum drift from FP64 diagnostics"),
+    "barneshut262k": dict(n=262144, desc="Plummer sphere N=262144 (a=1, M=1, G=1), dt=1e-3, "
+                                         "collisions off: the reference's Barnes-Hut octree "
+                                         "(theta 0.3, softening 1e-4, Build wired in) in FP64 "
+                                         "against the FP64 direct sum with the same softening"),
     "batch512": dict(n=0, desc="512 independent simulations, N uniform in [256, 8192] (seeded), "
                                "uniform-sphere clusters with lognormal masses, eps=0.01, "
                                "dt=1e-3, FP32 force + Kahan, FP64 update, one hipGraph launch "
@@ -279,6 +283,55 @@ def run_collide(a, w, pinned):
     print(json.dumps(res))
 
 
+PEAK_FP64 = 78.6e12      # MI355X FP64 vector peak (FLOP/s)
+FLOPS_PER_BH_TERM = 23   # 3 sub, 3 mul + 3 add (softened r^2), sqrt, 2h and /dist (opening
+                         # test), G*m*M and /r^2, 3 x (*d, /dist), 3 add; sqrt and / count as 1
+
+
+def run_barneshut(a, w):
+    """The reference's own far-field design: the octree with Build wired in, FP64, against the
+    FP64 direct sum with the same softening at the same N."""
+    from nbodygo_amd import NBodyGPU, gen
+    n = w["n"]
+    b = gen.plummer(n, seed=2024)
+    g = NBodyGPU(G=1.0, eps=1e-4, overlap_cutoff=False, precision="fp64", collisions=False)
+    g.load(b)
+    g.bh_enable()
+    s, el = timed_run(g, b, a.steps, a.warmup)
+    st = g.bh_stats()
+    ms_step = el / a.steps * 1e3
+    launch_ms = s["force_ms"] / max(s["force_launches"], 1)
+    terms = st["interactions"]
+    achieved = FLOPS_PER_BH_TERM * terms / (launch_ms * 1e-3)
+    # force error: the octree force against FP64 direct-sum accelerations of a sample
+    g.load(b)
+    idx = np.sort(np.random.default_rng(0).choice(n, 2048, replace=False))
+    f = g.forces()[idx] / b.m[idx, None]
+    ref = g.accel_fp64(idx)
+    err = float(np.linalg.norm(f - ref) / np.linalg.norm(ref))
+    g.close()
+    h = NBodyGPU(G=1.0, eps=1e-4, overlap_cutoff=False, precision="fp64", collisions=False)
+    sd, eld = timed_run(h, b, 2, 1)
+    h.close()
+    direct_ms = eld / 2 * 1e3
+    print(json.dumps({
+        "metric": METRIC, "value": n * (n - 1) / (ms_step * 1e-3) / 1e9,
+        "unit": "G interactions/s (N(N-1) per step, Barnes-Hut)", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
+        "steps_per_s": 1e3 / ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64", "data": "synthetic (seeded Plummer sphere)",
+        "config": {"workload": w["desc"], "n": n, "theta": 0.3, "softening": 1e-4},
+        "tree": {"terms_per_step": terms, "terms_per_body": terms / n, "cells": st["nodes"],
+                 "depth_max": st["depth_max"], "G_terms_per_s": terms / (launch_ms * 1e-3) / 1e9},
+        "force_error_vs_fp64_direct": err,
+        "fp64_direct": {"ms_per_step": direct_ms, "speedup_of_tree": direct_ms / ms_step},
+        "roofline": {"bound": "valu-fp64", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                     "kernel": "k_bh_force + k_bh_sun (tree walk)",
+                     "flops_per_term": FLOPS_PER_BH_TERM, "avg_launch_ms": launch_ms},
+    }))
+
+
 def cpu_baseline(b, seconds):
     """The reference's worker-pool CPU algorithm (oracle restatement) on a bounded sample: the
     first T targets of the same body set against all N sources, one Compute phase."""
@@ -312,6 +365,8 @@ def main():
         return run_batch(a, w)
     if a.workload == "collide2x524k":
         return run_collide(a, w, shape_kw(a.shape))
+    if a.workload == "barneshut262k":
+        return run_barneshut(a, w)
     if a.workload == "surrogate262k":
         sur = surrogate_leg(a, shape_kw(a.shape))
         n = sur["n"]

@@ -74,7 +74,12 @@ typedef struct nb_params {
   int32_t block;          /* FP32 force kernel: threads per workgroup (64..256); 0 = tuned */
   int32_t variant;        /* FP32 force kernel: 0 = tuned, 1 = scalar-cache j stream,
                              2 = LDS-staged j tiles */
-  int32_t reserved[7];
+  int32_t events;         /* 1 = the reference's subsume and fragmentation events on the device
+                             (Compute's dispatch, body.go:172-183; ResolveSubsume :228-244;
+                             doFragment / fragment, fragcalc.go:54-117); 0 = elastic collisions
+                             only (Subsume and None bodies make no pairs, shouldFragment is only
+                             counted) */
+  int32_t reserved[6];
 } nb_params;
 
 typedef struct nb_stats {
@@ -154,7 +159,7 @@ int nb_store(nb_ctx *ctx, double *x, double *y, double *z, double *vx, double *v
  * returned; any pointer may be NULL to skip it). */
 int nb_store_bodies(nb_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy,
                     double *vz, double *m, double *r, int32_t *beh, uint8_t *flags, double *rest,
-                    double *frag);
+                    double *frag, double *frag_step);
 
 /* FP32 positions for rendering (NewRenderable, renderable.go:22-40): xyz is 3*n floats;
  * exists (n bytes, may be NULL) receives the Exists flag. */
@@ -180,19 +185,22 @@ int nb_autotune(nb_ctx *ctx, int32_t reps, int32_t *cfg5);
  * computation-runner.go:176-216; BodyCollection.Cycle, body_collection.go:253-296), and the
  * GetBody / ModBody handshakes (body_collection.go:111-191) as single-body reads and patches. */
 typedef struct nb_body {
-  double x, y, z, vx, vy, vz, m, r, rest, frag;
+  double x, y, z, vx, vy, vz, m, r, rest, frag, frag_step;
   int32_t beh;
   uint8_t flags;
   uint8_t pad[3];
 } nb_body;
 
-/* Queue k bodies (arrays as in nb_load, frag may be NULL) to be appended by the next nb_cycle. */
+/* Queue k bodies (arrays as in nb_load; frag and frag_step may be NULL) to be appended by the
+ * next nb_cycle. */
 int nb_add_bodies(nb_ctx *ctx, int64_t k, const double *x, const double *y, const double *z,
                   const double *vx, const double *vy, const double *vz, const double *m,
-                  const double *r, const int32_t *beh, const uint8_t *flags, const double *frag);
+                  const double *r, const int32_t *beh, const uint8_t *flags, const double *frag,
+                  const double *frag_step);
 /* Cycle: drop the bodies that no longer exist (survivors keep their order), then append the
  * queued adds, newest first as the reference's event list holds them (body_collection.go:100,
- * 273-279), each with restitution R.  *n_out = the new body count. */
+ * 273-279), each with restitution R: first the fragments the device created since the last
+ * cycle (events on), then the bodies queued by nb_add_bodies.  *n_out = the new body count. */
 int nb_cycle(nb_ctx *ctx, double R, int64_t *n_out);
 /* processDeletes: count > 0 removes bodies at the reference's stride walk over the array
  * (existing, unpinned; the walk removes one more than asked in some cases, as there); count = -1
@@ -202,6 +210,36 @@ int nb_remove_bodies(nb_ctx *ctx, int64_t count, int64_t *removed);
 /* One body's state (GetBody) and a patch of it (ModBody / ApplyMods). */
 int nb_get_body(nb_ctx *ctx, int64_t index, nb_body *out);
 int nb_set_body(nb_ctx *ctx, int64_t index, const nb_body *in);
+
+/* ---- subsume and fragmentation events (nb_params.events = 1) -------------------------------- */
+/* Detection dispatches each overlapping pair as Compute does (body.go:172-183): a collision when
+ * both bodies are Elastic or Fragment, else a subsume when either is Subsume and the larger body
+ * (radii unequal) reaches the other's centre.  Pairs are resolved once each, in ascending (i, j)
+ * order: ResolveCollision as before, except that a true shouldFragment now starts fragmentation
+ * (doFragment / initiateFragmentation, fragcalc.go:54-83, with the reference's constants:
+ * fragment radius 1, at most 2000 fragments); ResolveSubsume (body.go:228-244) moves the smaller
+ * body's mass to the larger and removes it (Exists = 0, mass 0), with no existence check, as
+ * there.  Each cycle a fragmenting body creates up to 101 fragments (fragment, fragcalc.go:
+ * 97-117: position uniform in the ball of 0.9 times its radius, its velocity, mass / fragments,
+ * Elastic) into a device queue that nb_cycle appends; when none are left it stops existing.  A
+ * fragmenting body keeps the force of the cycle it started fragmenting in, as the reference's
+ * stale Body.fx does.  Fragment positions come from a counter-based generator (the reference
+ * seeds from the clock), so runs are reproducible. */
+/* Body.FragStep of every body (fragments per unit of impact factor above FragFactor,
+ * fragcalc.go:72); NULL sets 0.  nb_load sets 0; nb_add_bodies and nb_set_body carry it. */
+int nb_set_frag_steps(nb_ctx *ctx, const double *frag_step);
+
+typedef struct nb_event_stats {
+  int64_t subsumed_total;     /* ResolveSubsume calls carried out */
+  int64_t fragmentations;     /* bodies that started fragmenting */
+  int64_t fragments_emitted;  /* fragment bodies created */
+  int64_t fragments_pending;  /* created, waiting for nb_cycle */
+  int64_t fragmenting;        /* bodies fragmenting now */
+  int32_t queue_overflow;     /* 1 if a cycle deferred fragments for lack of queue space */
+  int32_t pad;
+} nb_event_stats;
+/* Totals since nb_load, as of the end of the last nb_step or nb_cycle (no device access). */
+int nb_get_event_stats(nb_ctx *ctx, nb_event_stats *out);
 
 /* ---- Barnes-Hut octree (the reference's own far-field design) ------------------------------ */
 /* cmd/body/octree.go with Build wired into BuildTree: the reference allocates an empty tree each

@@ -45,7 +45,8 @@ class NbParams(C.Structure):
         ("jsplit", C.c_int32),
         ("block", C.c_int32),
         ("variant", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("events", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -130,9 +131,18 @@ class NbBhStats(C.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
 
 
+class NbEventStats(C.Structure):
+    _fields_ = [("subsumed_total", C.c_int64), ("fragmentations", C.c_int64),
+                ("fragments_emitted", C.c_int64), ("fragments_pending", C.c_int64),
+                ("fragmenting", C.c_int64), ("queue_overflow", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
+
+
 class NbBody(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "r", "rest",
-                                           "frag")] + [
+                                           "frag", "frag_step")] + [
         ("beh", C.c_int32),
         ("flags", C.c_uint8),
         ("pad", C.c_uint8 * 3),
@@ -163,18 +173,20 @@ SIGNATURES = [
     ("nb_get_pairs", C.c_int, [_ctx, _i32p, C.c_int64, _i64p]),
     ("nb_store", C.c_int, [_ctx, _dp, _dp, _dp, _dp, _dp, _dp, _u8p, _dp]),
     ("nb_store_bodies", C.c_int,
-     [_ctx, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _u8p, _dp, _dp]),
+     [_ctx, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _u8p, _dp, _dp, _dp]),
     ("nb_store_render", C.c_int, [_ctx, _fp, _u8p]),
     ("nb_diagnostics", C.c_int, [_ctx, C.POINTER(NbDiag)]),
     ("nb_set_timing", C.c_int, [_ctx, C.c_int32]),
     ("nb_get_config", C.c_int, [_ctx, _i32p]),
     ("nb_autotune", C.c_int, [_ctx, C.c_int32, _i32p]),
     ("nb_add_bodies", C.c_int,
-     [_ctx, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _u8p, _dp]),
+     [_ctx, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _u8p, _dp, _dp]),
     ("nb_cycle", C.c_int, [_ctx, C.c_double, _i64p]),
     ("nb_remove_bodies", C.c_int, [_ctx, C.c_int64, _i64p]),
     ("nb_get_body", C.c_int, [_ctx, C.c_int64, C.POINTER(NbBody)]),
     ("nb_set_body", C.c_int, [_ctx, C.c_int64, C.POINTER(NbBody)]),
+    ("nb_set_frag_steps", C.c_int, [_ctx, _dp]),
+    ("nb_get_event_stats", C.c_int, [_ctx, C.POINTER(NbEventStats)]),
     ("nb_bh_params_init", C.c_int, [C.POINTER(NbBhParams)]),
     ("nb_bh_enable", C.c_int, [_ctx, C.POINTER(NbBhParams)]),
     ("nb_bh_disable", C.c_int, [_ctx]),

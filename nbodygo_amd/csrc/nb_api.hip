@@ -191,6 +191,7 @@ static int force_phase(Ctx *c, bool timed, int ev_slot) {
     a.eps = c->p.eps;
     a.cutoff = c->p.overlap_cutoff;
     a.collisions = c->p.collisions;
+    a.events = c->p.events;
     if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot], c->stream));
     NB_HIP(c, launch_force_f64(a, c->stream));
     if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], c->stream));
@@ -222,6 +223,7 @@ static int force_phase(Ctx *c, bool timed, int ev_slot) {
       f.recompute = a.drop_flagged;
       f.cutoff = c->p.overlap_cutoff;
       f.collisions = c->p.collisions;
+      f.events = c->p.events;
       NB_HIP(c, launch_fixup(f, c->stream));
     }
   }
@@ -275,6 +277,10 @@ int nb_create(const nb_params *p, nb_ctx **out) {
     g_create_err = "nb_create: eps must be finite and >= 0, G finite";
     return NB_E_INVALID;
   }
+  if (p->events != 0 && p->events != 1) {
+    g_create_err = "nb_create: events must be 0 or 1";
+    return NB_E_INVALID;
+  }
   if (p->precision != NB_PREC_FP32 && p->precision != NB_PREC_FP64) {
     g_create_err = "nb_create: unknown precision mode";
     return NB_E_INVALID;
@@ -316,6 +322,9 @@ void nb_destroy(nb_ctx *c) {
   c->sur = nullptr;
   bh_free(c->bh);
   c->bh = nullptr;
+  events_free(c);
+  if (c->ev_scan_tmp) (void)hipFree(c->ev_scan_tmp);
+  c->ev_scan_tmp = nullptr;
   free_all(c);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -392,6 +401,13 @@ int nb_load(nb_ctx *c, int64_t n, const double *x, const double *y, const double
   int rc = prepare_shape(c);
   if (rc) return rc;
   NB_HIP(c, launch_zero_counters(c->ctr, 1, s));
+  if (c->p.events) {
+    int rc2 = events_alloc(c, c->cap);
+    if (rc2) return rc2;
+    c->evs = nb_event_stats{};
+    rc2 = events_ensure_queue(c);
+    if (rc2) return rc2;
+  }
   NB_HIP(c, hipStreamSynchronize(s));
   c->n_active = n ? count_active(n, flags) : 0;
   c->rmax = 0;
@@ -422,7 +438,15 @@ int nb_step(nb_ctx *c, double dt, double R, int32_t nsteps, nb_stats *out) {
     }
   }
   const int64_t na0 = c->n_active;
+  if (c->p.events) {
+    int rc = events_ensure_queue(c);
+    if (rc) return rc;
+  }
   for (int s = 0; s < nsteps && c->n > 0; s++) {
+    if (c->p.events) {  // fragment() of the fragmenting bodies, in Compute (body.go:152-154)
+      int rc = events_emit(c);
+      if (rc) return rc;
+    }
     int rc = force_phase(c, timed, s);
     if (rc) return rc;
     if (c->p.collisions) {
@@ -435,6 +459,12 @@ int nb_step(nb_ctx *c, double dt, double R, int32_t nsteps, nb_stats *out) {
       ra.pairs = c->pairs;
       ra.ctr = c->ctr;
       ra.pair_cap = c->pair_cap;
+      ra.events = c->p.events;
+      ra.fstep = c->fstep;
+      ra.fst = c->fst;
+      ra.acc = c->acc;
+      ra.split = split_now(c);
+      ra.npad = (int)c->npad;
       NB_HIP(c, launch_resolve(ra, c->stream));
     }
     UpdateArgs u;
@@ -453,6 +483,7 @@ int nb_step(nb_ctx *c, double dt, double R, int32_t nsteps, nb_stats *out) {
     u.force_mode = c->p.precision == NB_PREC_FP64 ? 1 : 0;
     u.js = c->js;
     u.ctr = c->ctr;
+    u.fst = c->p.events ? c->fst : nullptr;
     NB_HIP(c, launch_update(u, c->stream));
   }
   DevCounters after;
@@ -467,8 +498,10 @@ int nb_step(nb_ctx *c, double dt, double R, int32_t nsteps, nb_stats *out) {
       force_ms += ms;
     }
   }
-  const int64_t removed = (int64_t)(after.removed_total - before.removed_total);
+  const int64_t removed = (int64_t)(after.removed_total - before.removed_total) +
+                          (int64_t)(after.inactive_events - before.inactive_events);
   c->n_active = std::max<int64_t>(0, c->n_active - removed);
+  if (c->p.events) events_refresh_host(c, after);
   if (out) {
     memset(out, 0, sizeof *out);
     out->steps = nsteps;
@@ -598,7 +631,7 @@ int nb_store(nb_ctx *c, double *x, double *y, double *z, double *vx, double *vy,
 
 int nb_store_bodies(nb_ctx *c, double *x, double *y, double *z, double *vx, double *vy,
                     double *vz, double *m, double *r, int32_t *beh, uint8_t *flags, double *rest,
-                    double *frag) {
+                    double *frag, double *frag_step) {
   if (!c) return NB_E_INVALID;
   if (!c->loaded) return fail(c, NB_E_STATE, "nb_store_bodies: no bodies loaded");
   if (c->n == 0) return NB_OK;
@@ -612,6 +645,12 @@ int nb_store_bodies(nb_ctx *c, double *x, double *y, double *z, double *vx, doub
     NB_HIP(c, hipMemcpyAsync(beh, c->beh, (size_t)c->n * sizeof(int32_t), hipMemcpyDeviceToHost,
                              c->stream));
   if (flags) NB_HIP(c, hipMemcpyAsync(flags, c->flags, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+  if (frag_step) {
+    if (c->fstep)
+      NB_HIP(c, hipMemcpyAsync(frag_step, c->fstep, db, hipMemcpyDeviceToHost, c->stream));
+    else
+      memset(frag_step, 0, db);
+  }
   NB_HIP(c, hipStreamSynchronize(c->stream));
   return NB_OK;
 }
@@ -769,7 +808,8 @@ extern "C" {
 
 int nb_add_bodies(nb_ctx *c, int64_t k, const double *x, const double *y, const double *z,
                   const double *vx, const double *vy, const double *vz, const double *m,
-                  const double *r, const int32_t *beh, const uint8_t *flags, const double *frag) {
+                  const double *r, const int32_t *beh, const uint8_t *flags, const double *frag,
+                  const double *frag_step) {
   if (!c) return NB_E_INVALID;
   if (k < 0 || k > (int64_t)1 << 30) return fail(c, NB_E_INVALID, "nb_add_bodies: bad count");
   if (k > 0 && (!x || !y || !z || !vx || !vy || !vz || !m || !r || !beh || !flags))
@@ -778,6 +818,7 @@ int nb_add_bodies(nb_ctx *c, int64_t k, const double *x, const double *y, const 
   auto &A = c->adds;
   for (int q = 0; q < 10; q++)
     for (int64_t i = 0; i < k; i++) A.a[q].push_back(src[q] ? src[q][i] : (q == 8 ? 1.0 : 0.0));
+  for (int64_t i = 0; i < k; i++) A.fstep.push_back(frag_step ? frag_step[i] : 0.0);
   A.beh.insert(A.beh.end(), beh, beh + k);
   A.flags.insert(A.flags.end(), flags, flags + k);
   return NB_OK;
@@ -819,29 +860,43 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
     NB_HIP(c, hipStreamSynchronize(st));
     nk = (int64_t)lp + lk;
   }
-  if (nk == n && nadd == 0) {  // nothing removed, nothing added
+  int64_t nq = 0;  // fragments waiting in the device queue
+  if (c->p.events) {
+    DevCounters h;
+    NB_HIP(c, hipMemcpyAsync(&h, c->ctr, sizeof h, hipMemcpyDeviceToHost, st));
+    NB_HIP(c, hipStreamSynchronize(st));
+    nq = (int64_t)h.frag_pending;
+  }
+  if (nk == n && nadd == 0 && nq == 0) {  // nothing removed, nothing added
     if (n_out) *n_out = n;
     return NB_OK;
   }
-  const int64_t n_new = nk + nadd;
+  const int64_t n_new = nk + nq + nadd;
   if (n_new > (int64_t)1 << 30) return fail(c, NB_E_INVALID, "nb_cycle: too many bodies");
   const int64_t npad_new = std::max<int64_t>(round_up(n_new, kPad), kPad);
   const int64_t cap_new = std::max<int64_t>(c->cap, npad_new);
   double *d[10] = {};
   int32_t *beh = nullptr;
   uint8_t *fl = nullptr, *col = nullptr;
+  double *fs = nullptr;
+  FragState *fst = nullptr;
   auto release = [&]() {
     for (auto *q : d)
       if (q) (void)hipFree(q);
     if (beh) (void)hipFree(beh);
     if (fl) (void)hipFree(fl);
     if (col) (void)hipFree(col);
+    if (fs) (void)hipFree(fs);
+    if (fst) (void)hipFree(fst);
   };
   bool ok = true;
   for (int q = 0; q < 10 && ok; q++) ok = hipMalloc((void **)&d[q], (size_t)cap_new * 8) == hipSuccess;
   ok = ok && hipMalloc((void **)&beh, (size_t)cap_new * 4) == hipSuccess &&
        hipMalloc((void **)&fl, (size_t)cap_new) == hipSuccess &&
        hipMalloc((void **)&col, (size_t)cap_new) == hipSuccess;
+  if (ok && c->p.events)
+    ok = hipMalloc((void **)&fs, (size_t)cap_new * 8) == hipSuccess &&
+         hipMalloc((void **)&fst, (size_t)cap_new * sizeof(FragState)) == hipSuccess;
   if (!ok) {
     release();
     return fail(c, NB_E_NOMEM, "nb_cycle: device allocation failed");
@@ -852,6 +907,8 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
   if (e == hipSuccess) e = hipMemsetAsync(beh, 0, (size_t)cap_new * 4, st);
   if (e == hipSuccess) e = hipMemsetAsync(fl, 0, (size_t)cap_new, st);
   if (e == hipSuccess) e = hipMemsetAsync(col, 0, (size_t)cap_new, st);
+  if (e == hipSuccess && fs) e = hipMemsetAsync(fs, 0, (size_t)cap_new * 8, st);
+  if (e == hipSuccess && fst) e = hipMemsetAsync(fst, 0, (size_t)cap_new * sizeof(FragState), st);
   CompactArgs ca;
   const double *olds[10] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m, c->r, c->rest, c->frag};
   for (int q = 0; q < 10; q++) {
@@ -863,8 +920,14 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
   ca.flags_src = c->flags;
   ca.flags_dst = fl;
   ca.pos = pos;
+  ca.fstep_src = c->fstep;
+  ca.fstep_dst = fs;
+  ca.fst_src = c->fst;
+  ca.fst_dst = fst;
   ca.n = (int)n;
   if (e == hipSuccess) e = launch_compact(ca, st);
+  // the device's fragments first (the newest events), then the host's queue
+  if (e == hipSuccess && nq) e = launch_append_frags(c, nq, d, beh, fl, fs, fst, nk, R, st);
   // queued adds go after the survivors, newest first, each with restitution R: the reference
   // keeps its events with PushFront (body_collection.go:100) and Cycle walks them from the front
   // (body_collection.go:273-279, 286-290)
@@ -873,12 +936,16 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
   for (int q = 0; q < 10; q++) std::reverse(A.a[q].begin(), A.a[q].end());
   std::reverse(A.beh.begin(), A.beh.end());
   std::reverse(A.flags.begin(), A.flags.end());
+  std::reverse(A.fstep.begin(), A.fstep.end());
+  const int64_t at = nk + nq;
   for (int q = 0; q < 10 && e == hipSuccess && nadd; q++)
-    e = hipMemcpyAsync(d[q] + nk, A.a[q].data(), (size_t)nadd * 8, hipMemcpyHostToDevice, st);
+    e = hipMemcpyAsync(d[q] + at, A.a[q].data(), (size_t)nadd * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess && nadd)
-    e = hipMemcpyAsync(beh + nk, A.beh.data(), (size_t)nadd * 4, hipMemcpyHostToDevice, st);
+    e = hipMemcpyAsync(beh + at, A.beh.data(), (size_t)nadd * 4, hipMemcpyHostToDevice, st);
   if (e == hipSuccess && nadd)
-    e = hipMemcpyAsync(fl + nk, A.flags.data(), (size_t)nadd, hipMemcpyHostToDevice, st);
+    e = hipMemcpyAsync(fl + at, A.flags.data(), (size_t)nadd, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && nadd && fs)
+    e = hipMemcpyAsync(fs + at, A.fstep.data(), (size_t)nadd * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     release();
@@ -897,6 +964,12 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
   c->beh = beh;
   c->flags = fl;
   c->collided = col;
+  if (c->p.events) {
+    if (c->fstep) (void)hipFree(c->fstep);
+    if (c->fst) (void)hipFree(c->fst);
+    c->fstep = fs;
+    c->fst = fst;
+  }
   if (cap_new > c->cap) {
     c->cap = cap_new;
     NB_HIP(c, dalloc(&c->js, c->cap));
@@ -920,7 +993,14 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
     const uint8_t f = A.flags[(size_t)i];
     if ((f & NB_F_EXISTS) && !(f & NB_F_FRAGMENTING)) c->n_active++;
   }
+  c->n_active += nq;  // fragments exist and are not fragmenting
   A.clear();
+  if (c->p.events) {
+    DevCounters h;
+    NB_HIP(c, hipMemcpyAsync(&h, c->ctr, sizeof h, hipMemcpyDeviceToHost, st));
+    NB_HIP(c, hipStreamSynchronize(st));
+    events_refresh_host(c, h);
+  }
   rc = refresh_rmax(c);
   if (rc) return rc;
   if (n_out) *n_out = n_new;
@@ -969,6 +1049,9 @@ int nb_get_body(nb_ctx *c, int64_t index, nb_body *out) {
     NB_HIP(c, hipMemcpyAsync(dst[q], src[q] + index, 8, hipMemcpyDeviceToHost, c->stream));
   NB_HIP(c, hipMemcpyAsync(&out->beh, c->beh + index, 4, hipMemcpyDeviceToHost, c->stream));
   NB_HIP(c, hipMemcpyAsync(&out->flags, c->flags + index, 1, hipMemcpyDeviceToHost, c->stream));
+  if (c->fstep)
+    NB_HIP(c, hipMemcpyAsync(&out->frag_step, c->fstep + index, 8, hipMemcpyDeviceToHost,
+                             c->stream));
   NB_HIP(c, hipStreamSynchronize(c->stream));
   return NB_OK;
 }
@@ -987,6 +1070,9 @@ int nb_set_body(nb_ctx *c, int64_t index, const nb_body *in) {
     NB_HIP(c, hipMemcpyAsync(dst[q] + index, src[q], 8, hipMemcpyHostToDevice, c->stream));
   NB_HIP(c, hipMemcpyAsync(c->beh + index, &in->beh, 4, hipMemcpyHostToDevice, c->stream));
   NB_HIP(c, hipMemcpyAsync(c->flags + index, &in->flags, 1, hipMemcpyHostToDevice, c->stream));
+  if (c->fstep)
+    NB_HIP(c, hipMemcpyAsync(c->fstep + index, &in->frag_step, 8, hipMemcpyHostToDevice,
+                             c->stream));
   NB_HIP(c, launch_write_stream(c->x, c->y, c->z, c->m, c->flags, c->js, (int)c->n,
                                 (int)c->npad, c->stream));
   NB_HIP(c, launch_rmax_tile(c->r, c->flags, (int)c->n, c->shape.tile_j, tiles_of(c),

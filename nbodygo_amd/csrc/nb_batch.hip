@@ -320,7 +320,7 @@ int nb_batch_create(const nb_params *p, nb_batch **out) {
   *out = nullptr;
   if (!(p->eps > 0.0) || !std::isfinite(p->eps) || (float)(p->eps * p->eps) < 1e-30f)
     return NB_E_INVALID;  // the batched path is the softened FP32 force
-  if (p->collisions || p->overlap_cutoff || p->precision != NB_PREC_FP32)
+  if (p->collisions || p->overlap_cutoff || p->events || p->precision != NB_PREC_FP32)
     return NB_E_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || p->device < 0 || p->device >= ndev)

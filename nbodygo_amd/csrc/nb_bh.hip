@@ -95,10 +95,6 @@ __device__ __forceinline__ bool bh_active(uint8_t f) {
   return (f & NB_F_EXISTS) && !(f & NB_F_FRAGMENTING);
 }
 
-__device__ __forceinline__ bool bh_collide_b(int32_t b) {
-  return b == NB_CB_ELASTIC || b == NB_CB_FRAGMENT;
-}
-
 __device__ __forceinline__ void bh_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
@@ -644,6 +640,7 @@ struct BhDetectArgs {
   DevCounters *ctr;
   int64_t pair_cap;
   double rmax;
+  int events;
 };
 
 __device__ __forceinline__ void bh_emit(const BhDetectArgs &a, int i, int j) {
@@ -654,16 +651,17 @@ __device__ __forceinline__ void bh_emit(const BhDetectArgs &a, int i, int j) {
     atomicOr(&a.ctr->pair_overflow, 1u);
 }
 
-// Collided (body.go:192-208) from body i's side, pairs i < j with both behaviours colliding.
-__device__ __forceinline__ void bh_test_pair(const BhDetectArgs &a, int i, double xi, double yi,
-                                             double zi, double ri, int s2) {
+// Collided (body.go:192-208) from body i's side, pairs i < j, dispatched as Compute does.
+__device__ __forceinline__ void bh_test_pair(const BhDetectArgs &a, int i, int32_t bi, double xi,
+                                             double yi, double zi, double ri, int s2) {
   const int j = a.perm[s2];
-  if (j <= i || !bh_collide_b(a.beh[j])) return;
+  if (j <= i) return;
   const double dx = a.x[j] - xi;
   const double dy = a.y[j] - yi;
   const double dz = a.z[j] - zi;
   const double dist = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
-  if (dist <= ri + a.r[j]) bh_emit(a, i, j);
+  const double rj = a.r[j];
+  if (dist <= ri + rj && nb_pair_event(bi, a.beh[j], ri, rj, dist, a.events)) bh_emit(a, i, j);
 }
 
 __global__ __launch_bounds__(256) void k_bh_detect(BhDetectArgs a) {
@@ -676,10 +674,12 @@ __global__ __launch_bounds__(256) void k_bh_detect(BhDetectArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   int i = -1;
   bool act = false;
+  int32_t bi = NB_CB_NONE;
   double xi = 0, yi = 0, zi = 0, ri = 0, lim = 0;
   if (s < M) {
     i = a.perm[s];
-    act = bh_collide_b(a.beh[i]);
+    bi = a.beh[i];
+    act = nb_may_pair(bi, a.events);
     xi = a.x[i];
     yi = a.y[i];
     zi = a.z[i];
@@ -702,14 +702,14 @@ __global__ __launch_bounds__(256) void k_bh_detect(BhDetectArgs a) {
     const unsigned long long msk = stk_mask[w][sp];
     const bool on = (msk >> lane) & 1ull;
     if (code <= -2) {
-      if (on) bh_test_pair(a, i, xi, yi, zi, ri, -2 - code);
+      if (on) bh_test_pair(a, i, bi, xi, yi, zi, ri, -2 - code);
       continue;
     }
     const int32_t *ch = a.nchild + (size_t)code * 8;
     if (ch[0] == kBucket) {
       if (on)
         for (int s2 = ch[1]; s2 < ch[1] + ch[2]; s2++)
-          if (s2 != s) bh_test_pair(a, i, xi, yi, zi, ri, s2);
+          if (s2 != s) bh_test_pair(a, i, bi, xi, yi, zi, ri, s2);
       continue;
     }
     const double4 bx = a.nbox[code];
@@ -903,6 +903,7 @@ int bh_force_phase(Ctx *c, bool timed, int ev_slot) {
     da.ctr = c->ctr;
     da.pair_cap = c->pair_cap;
     da.rmax = c->rmax;
+    da.events = c->p.events;
     hipLaunchKernelGGL(k_bh_detect, dim3(nb256(n)), dim3(256), 0, st, da);
     NB_HIP(c, hipGetLastError());
     NB_HIP(c, launch_sort_pairs(c->pairs, c->pair_cap, c->ctr, st));

@@ -46,7 +46,8 @@ __global__ __launch_bounds__(256) void k_force_f64(Force64Args a) {
   if (__builtin_amdgcn_ballot_w64(ai) != 0) {
     const double xi = ai ? a.x[i] : 0, yi = ai ? a.y[i] : 0, zi = ai ? a.z[i] : 0;
     const double mi = ai ? a.m[i] : 0, ri = ai ? a.r[i] : 0;
-    const bool ci = ai && collide_b(a.beh[i]);
+    const int32_t bi = ai ? a.beh[i] : NB_CB_NONE;
+    const bool ci = ai && nb_may_pair(bi, a.events);
     const double G = a.G, eps2 = a.eps * a.eps;
     const bool ref = a.eps == 0.0 && a.cutoff;
     cdp X = (cdp)a.x, Y = (cdp)a.y, Z = (cdp)a.z, M = (cdp)a.m, Rr = (cdp)a.r;
@@ -59,18 +60,16 @@ __global__ __launch_bounds__(256) void k_force_f64(Force64Args a) {
       const double d2 = dx * dx + dy * dy + dz * dz;
       const double rs = ri + Rr[j];
       if (j != i && ai) {
-        bool overlap;
+        const double dist0 = __builtin_sqrt(d2);
+        const bool overlap = !(dist0 > rs);
         if (ref) {
-          const double dist = __builtin_sqrt(d2);
-          overlap = !(dist > rs);
           if (!overlap) {
-            const double force = G * mi * M[j] / (dist * dist);
-            fx += force * dx / dist;
-            fy += force * dy / dist;
-            fz += force * dz / dist;
+            const double force = G * mi * M[j] / (dist0 * dist0);
+            fx += force * dx / dist0;
+            fy += force * dy / dist0;
+            fz += force * dz / dist0;
           }
         } else {
-          overlap = !(__builtin_sqrt(d2) > rs);
           if (!(a.cutoff && overlap)) {
             const double distSq = d2 + eps2;
             const double dist = __builtin_sqrt(distSq);
@@ -80,7 +79,8 @@ __global__ __launch_bounds__(256) void k_force_f64(Force64Args a) {
             fz += force * dz / dist;
           }
         }
-        if (a.collisions && overlap && i < j && ci && collide_b(B[j]))
+        if (a.collisions && overlap && i < j && ci &&
+            nb_pair_event(bi, B[j], ri, Rr[j], dist0, a.events))
           emit(a.pairs, a.ctr, a.pair_cap, i, j);
       }
     }
@@ -289,9 +289,9 @@ __device__ static bool elastic(const ResolveArgs &a, int b, int o, Elastic &res)
   return true;
 }
 
-// shouldFragment (fragcalc.go:24-49).  Fragmentation itself is outside the ported path: a true
-// result is counted and the velocities are left unchanged.
-__device__ static bool should_fragment(const ResolveArgs &a, int b, int o, const Elastic &e) {
+// shouldFragment (fragcalc.go:24-49); the factors go to doFragment.
+__device__ static bool should_fragment(const ResolveArgs &a, int b, int o, const Elastic &e,
+                                       double *tf, double *of) {
   if (!(a.beh[b] == NB_CB_FRAGMENT || a.beh[o] == NB_CB_FRAGMENT)) return false;
   const double br = a.rest[b];
   const double vThis = a.vx[b] + a.vy[b] + a.vz[b];
@@ -305,18 +305,71 @@ __device__ static bool should_fragment(const ResolveArgs &a, int b, int o, const
                          fabs(a.vz[o] - ((e.v2[2] - e.cm[2]) * br + e.cm[2]));
   const double otherFactor = dvOther / fabs(vOther);
   const double fb = a.frag[b], fo = a.frag[o];
+  *tf = thisFactor;
+  *of = otherFactor;
   return (a.beh[b] == NB_CB_FRAGMENT && thisFactor > fb) ||
          (a.beh[o] == NB_CB_FRAGMENT && otherFactor > fo);
+}
+
+// initiateFragmentation (fragcalc.go:65-83): fragDelta, the fragment count Min(delta * FragStep,
+// 2000), and fragInfo.  The body keeps this cycle's force while it fragments.
+__device__ static void initiate_fragmentation(const ResolveArgs &a, int b, double factor,
+                                              uint64_t *started, uint64_t *inactive) {
+  const double fragDelta = a.frag[b] > 10 ? 10.0 : factor - a.frag[b];
+  const double prod = fragDelta * a.fstep[b];
+  const double fragments = isnan(prod) ? prod : (prod < 2000.0 ? prod : 2000.0);  // math.Min
+  if (fragments <= 1) {
+    a.beh[b] = NB_CB_FRAGMENT;
+    return;
+  }
+  const uint8_t f = a.flags[b];
+  if (!(f & NB_F_FRAGMENTING)) {
+    (*started)++;
+    if (f & NB_F_EXISTS) (*inactive)++;
+  }
+  a.flags[b] = (uint8_t)(f | NB_F_FRAGMENTING);
+  FragState st;
+  double fx = 0, fy = 0, fz = 0;
+  for (int sl = 0; sl < a.split; sl++) {
+    const double *o = a.acc + (size_t)sl * 3 * a.npad;
+    fx += o[b];
+    fy += o[a.npad + b];
+    fz += o[2 * a.npad + b];
+  }
+  st.fx = fx;
+  st.fy = fy;
+  st.fz = fz;
+  st.mass = a.m[b] / fragments;
+  st.radius = a.r[b];
+  st.px = a.x[b];
+  st.py = a.y[b];
+  st.pz = a.z[b];
+  // int(fragments): truncation; NaN converts to the most negative int64 on amd64
+  st.left = isnan(fragments) ? INT64_MIN : (int64_t)fragments;
+  st.made = 0;
+  st.id = (int64_t)(a.ctr->frag_seq++);
+  a.fst[b] = st;
 }
 
 __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   if (threadIdx.x != 0) return;
   const uint32_t found = a.ctr->pair_count;
   const int64_t cnt = min((int64_t)found, a.pair_cap);
-  uint64_t resolved = 0, fragreq = 0;
+  uint64_t resolved = 0, fragreq = 0, subsumed = 0, started = 0, inactive = 0;
   for (int64_t k = 0; k < cnt; k++) {
     const uint64_t key = a.pairs[k];
     const int lo = (int)(key >> 32), hi = (int)(key & 0xffffffffu);
+    if (a.events && !(collide_b(a.beh[lo]) && collide_b(a.beh[hi]))) {
+      // a subsume pair: ResolveSubsume (body.go:228-244) by the larger body, no existence check
+      const int b = a.r[lo] > a.r[hi] ? lo : hi, o = b == lo ? hi : lo;
+      a.m[b] = a.m[b] + a.m[o];
+      const uint8_t fo = a.flags[o];
+      if ((fo & NB_F_EXISTS) && !(fo & NB_F_FRAGMENTING)) inactive++;
+      a.flags[o] = (uint8_t)(fo & ~NB_F_EXISTS);  // SetNotExists (body.go:93-96)
+      a.m[o] = 0.0;
+      subsumed++;
+      continue;
+    }
     int b, o;
     if (a.beh[lo] == NB_CB_ELASTIC && collide_b(a.beh[hi])) {
       b = lo;
@@ -331,8 +384,15 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
     Elastic e;
     if (!elastic(a, b, o, e)) continue;
     resolved++;
-    if (should_fragment(a, b, o, e)) {
+    double tf = 0, of = 0;
+    if (should_fragment(a, b, o, e, &tf, &of)) {
       fragreq++;
+      if (a.events) {  // doFragment (fragcalc.go:54-63)
+        if (a.beh[b] == NB_CB_FRAGMENT && tf > a.frag[b])
+          initiate_fragmentation(a, b, tf, &started, &inactive);
+        if (a.beh[o] == NB_CB_FRAGMENT && of > a.frag[o])
+          initiate_fragmentation(a, o, of, &started, &inactive);
+      }
       continue;
     }
     // doElastic (collisioncalc.go:26-35): restitution of the first body.
@@ -350,6 +410,9 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   a.ctr->pairs_last = found;
   a.ctr->resolved_total += resolved;
   a.ctr->fragment_requests += fragreq;
+  a.ctr->subsumed_total += subsumed;
+  a.ctr->frag_started += started;
+  a.ctr->inactive_events += inactive;
 }
 
 hipError_t launch_resolve(const ResolveArgs &a, hipStream_t st) {
@@ -382,11 +445,17 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a) {
   if (!a.collided[i]) {
     const double mi = a.m[i];
     double fx = 0, fy = 0, fz = 0;
-    for (int s = 0; s < a.split; s++) {
-      const double *o = a.acc + (size_t)s * 3 * a.npad;
-      fx += o[i];
-      fy += o[a.npad + i];
-      fz += o[2 * a.npad + i];
+    if (a.fst && (f & NB_F_FRAGMENTING)) {  // the force Body.fx kept from its last Compute
+      fx = a.fst[i].fx;
+      fy = a.fst[i].fy;
+      fz = a.fst[i].fz;
+    } else {
+      for (int s = 0; s < a.split; s++) {
+        const double *o = a.acc + (size_t)s * 3 * a.npad;
+        fx += o[i];
+        fy += o[a.npad + i];
+        fz += o[2 * a.npad + i];
+      }
     }
     if (a.force_mode == 1) {
       // FP64 path: acc holds the force; Vx += timeScaling * fx / Mass (body.go:120-122).
@@ -644,6 +713,8 @@ __global__ void k_compact(CompactArgs a) {
   for (int k = 0; k < 10; k++) a.dst[k][p] = a.src[k][i];
   a.beh_dst[p] = a.beh_src[i];
   a.flags_dst[p] = a.flags_src[i];
+  if (a.fstep_dst) a.fstep_dst[p] = a.fstep_src[i];
+  if (a.fst_dst) a.fst_dst[p] = a.fst_src[i];
 }
 
 hipError_t launch_compact(const CompactArgs &a, hipStream_t st) {

@@ -321,7 +321,8 @@ __global__ __launch_bounds__(256) void k_fixup(FixupArgs a) {
     const bool all = cnt > kFlagSlots;
     const int ntile = all ? (t1 - t0) : cnt;
     const double xi = a.x[i], yi = a.y[i], zi = a.z[i], ri = a.r[i];
-    const bool ci = can_collide(a.beh[i]);
+    const int32_t bi = a.beh[i];
+    const bool ci = nb_may_pair(bi, a.events);
     double ax = 0, ay = 0, az = 0;
     for (int u = 0; u < ntile; u++) {
       int t;
@@ -335,8 +336,11 @@ __global__ __launch_bounds__(256) void k_fixup(FixupArgs a) {
         const double dy = a.y[j] - yi;
         const double dz = a.z[j] - zi;
         const double d2 = dx * dx + dy * dy + dz * dz;
-        const bool overlap = !(__builtin_sqrt(d2) > ri + a.r[j]);
-        if (overlap && a.collisions && i < j && ci && can_collide(a.beh[j]))
+        const double dist = __builtin_sqrt(d2);
+        const double rj = a.r[j];
+        const bool overlap = !(dist > ri + rj);
+        if (overlap && a.collisions && i < j && ci &&
+            nb_pair_event(bi, a.beh[j], ri, rj, dist, a.events))
           emit_pair(a.pairs, a.ctr, a.pair_cap, i, j);
         if (a.recompute && !(overlap && a.cutoff)) {
           const double ds = d2 + a.eps2;

@@ -32,7 +32,47 @@ struct DevCounters {
   uint32_t pair_overflow;
   uint32_t pad;
   uint64_t sur_fallbacks;    // surrogate steps redone by direct summation
+  // subsume and fragmentation events (nb_params.events)
+  uint64_t subsumed_total;   // ResolveSubsume calls
+  uint64_t frag_started;     // bodies that started fragmenting
+  uint64_t frag_finished;    // fragmenting bodies that ran out of fragments
+  uint64_t frag_emitted;     // fragments created
+  uint64_t frag_seq;         // next fragmentation id (order of initiation)
+  uint64_t inactive_events;  // active bodies made inactive by a subsume or a fragmentation
+  uint32_t frag_pending;     // fragments in the queue, waiting for nb_cycle
+  uint32_t frag_overflow;    // a cycle deferred its fragments (queue full)
 };
+
+// Fragmentation bookkeeping of one body (Body.fragInfo, fragcalc.go:75-83, and the force the body
+// keeps while fragmenting).
+struct FragState {
+  double fx, fy, fz;    // force (or acceleration / G on the FP32 path) of the starting cycle
+  double mass, radius;  // fragment mass; the body's radius when it started
+  double px, py, pz;    // its position then (fragInfo.curPos)
+  int64_t left;         // fragments still to create
+  int64_t made;         // fragments created so far
+  int64_t id;           // fragmentation id (seeds the fragment positions)
+};
+
+constexpr int kFragPerCycle = 101;  // fragment() creates until cnt > maxFragsPerCycle (100)
+constexpr double kFragRadius = 1.0; // fragment radius: Pow(x, 1/3) with 1/3 == 0 (fragcalc.go:81)
+
+__device__ __forceinline__ bool nb_collide_b(int32_t b) {
+  return b == NB_CB_ELASTIC || b == NB_CB_FRAGMENT;
+}
+// Compute's dispatch of an overlapping pair (body.go:172-183): a collision event when both bodies
+// are Elastic or Fragment; with events on, otherwise a subsume event when either body is Subsume
+// and the larger one (radii unequal) reaches the other's centre (dist <= its radius).
+__device__ __forceinline__ bool nb_pair_event(int32_t bi, int32_t bj, double ri, double rj,
+                                              double dist, int events) {
+  if (nb_collide_b(bi) && nb_collide_b(bj)) return true;
+  if (!events || (bi != NB_CB_SUBSUME && bj != NB_CB_SUBSUME)) return false;
+  return (ri > rj && dist <= ri) || (rj > ri && dist <= rj);
+}
+// Whether a body can be in any event pair.
+__device__ __forceinline__ bool nb_may_pair(int32_t b, int events) {
+  return events || nb_collide_b(b);
+}
 
 struct Sur;  // far-field surrogate state (nb_sur.hip)
 struct Bh;   // Barnes-Hut octree state (nb_bh.hip)
@@ -96,14 +136,25 @@ struct Ctx {
   Bh *bh = nullptr;
   bool bh_on = false;
 
+  // events (nb_params.events): per-body FragStep and fragmentation state, the fragment queue
+  double *fstep = nullptr;
+  FragState *fst = nullptr;
+  double *fq = nullptr;  // queue: 8 arrays of fq_cap (x y z vx vy vz m r)
+  int64_t fq_cap = 0;
+  nb_event_stats evs{};  // host copy, refreshed at the end of nb_step and nb_cycle
+  void *ev_scan_tmp = nullptr;
+  size_t ev_scan_bytes = 0;
+
   // bodies queued by nb_add_bodies, appended by the next nb_cycle
   struct Adds {
     std::vector<double> a[10];  // x y z vx vy vz m r rest frag
+    std::vector<double> fstep;  // FragStep
     std::vector<int32_t> beh;
     std::vector<uint8_t> flags;
     int64_t count() const { return (int64_t)beh.size(); }
     void clear() {
       for (auto &v : a) v.clear();
+      fstep.clear();
       beh.clear();
       flags.clear();
     }
@@ -173,6 +224,7 @@ struct FixupArgs {
   int recompute;  // add FP64 contributions of flagged tiles (the fast path dropped them)
   int cutoff;     // overlap gate
   int collisions; // emit pairs
+  int events;     // subsume pairs too
 };
 hipError_t launch_fixup(const FixupArgs &a, hipStream_t st);
 
@@ -186,7 +238,7 @@ struct Force64Args {
   int64_t pair_cap;
   int n, npad;
   double G, eps;
-  int cutoff, collisions;
+  int cutoff, collisions, events;
 };
 hipError_t launch_force_f64(const Force64Args &a, hipStream_t st);
 
@@ -207,11 +259,17 @@ hipError_t launch_sort_pairs(uint64_t *keys, int64_t cap, const DevCounters *ctr
 
 struct ResolveArgs {
   double *x, *y, *z, *vx, *vy, *vz, *m, *r, *rest, *frag;
-  const int32_t *beh;
+  int32_t *beh;
   uint8_t *flags, *collided;
   const uint64_t *pairs;
   DevCounters *ctr;
   int64_t pair_cap;
+  // events: subsume, and fragmentation started from shouldFragment
+  int events;
+  const double *fstep;
+  FragState *fst;
+  const double *acc;  // this cycle's force slices, kept by a body that starts fragmenting
+  int split, npad;
 };
 hipError_t launch_resolve(const ResolveArgs &a, hipStream_t st);
 
@@ -226,6 +284,7 @@ struct UpdateArgs {
   int force_mode;  // 0: acc holds accelerations / G (FP32 path); 1: acc holds force (FP64 path)
   float4 *js;
   DevCounters *ctr;
+  const FragState *fst;  // events: a fragmenting body is kicked by its kept force
 };
 hipError_t launch_update(const UpdateArgs &a, hipStream_t st);
 
@@ -253,6 +312,10 @@ struct CompactArgs {
   const uint8_t *flags_src;
   uint8_t *flags_dst;
   const int32_t *pos;  // exclusive prefix sum of the keep flags
+  const double *fstep_src;  // events: FragStep and fragmentation state (may be null)
+  double *fstep_dst;
+  const FragState *fst_src;
+  FragState *fst_dst;
   int n;
 };
 hipError_t launch_keep_flags(const uint8_t *flags, int n, int32_t *keep, hipStream_t st);
@@ -261,6 +324,17 @@ hipError_t launch_rmax_active(const double *r, const uint8_t *flags, int n,
                               unsigned long long *out, hipStream_t st);
 hipError_t launch_remove_walk(uint8_t *flags, double *m, int n, int step, int npoints, int all,
                               unsigned long long *removed, hipStream_t st);
+
+// events (nb_events.hip)
+int events_emit(Ctx *c);                 // fragment() of every fragmenting body, one cycle
+int events_alloc(Ctx *c, int64_t cap);   // per-body arrays for cap bodies (zeroed)
+int events_ensure_queue(Ctx *c);         // queue room for the next cycles' fragments
+void events_free(Ctx *c);
+void events_refresh_host(Ctx *c, const DevCounters &d);
+// append the queue (newest first) at dst + at; returns fragments appended
+hipError_t launch_append_frags(Ctx *c, int64_t nq, double *const dst[10], int32_t *beh,
+                               uint8_t *flags, double *fstep, FragState *fst, int64_t at,
+                               double R, hipStream_t st);
 
 }  // namespace nb
 

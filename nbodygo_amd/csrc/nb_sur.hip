@@ -566,6 +566,7 @@ struct NearArgs {
   uint64_t *pairs;
   DevCounters *ctr;
   int64_t pair_cap;
+  int events;
 };
 
 __device__ __forceinline__ float round_up_f(double d) {  // smallest float >= d (d >= 0)
@@ -578,10 +579,6 @@ __device__ __forceinline__ float wave_maxf(float v) {
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
   return v;
-}
-
-__device__ __forceinline__ bool collides(int32_t b) {
-  return b == NB_CB_ELASTIC || b == NB_CB_FRAGMENT;
 }
 
 __device__ __forceinline__ void sur_emit_pair(const NearArgs &a, uint32_t i, uint32_t j) {
@@ -597,14 +594,16 @@ __device__ __forceinline__ void sur_emit_pair(const NearArgs &a, uint32_t i, uin
 // unordered pair is reported once, by its lower body index.
 __device__ void exact_chunk_pairs(const NearArgs &a, uint32_t i, int s0, int cnt) {
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i], ri = a.r[i];
-  const bool ci = collides(a.beh[i]);
-  if (!ci) return;
+  const int32_t bi = a.beh[i];
+  if (!nb_may_pair(bi, a.events)) return;
   for (int jj = 0; jj < cnt; jj++) {
     const uint32_t j = a.perm[s0 + jj];
     if (j <= i) continue;
     const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
     const double d = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
-    if (!(d > ri + a.r[j]) && collides(a.beh[j])) sur_emit_pair(a, i, j);
+    const double rj = a.r[j];
+    if (!(d > ri + rj) && nb_pair_event(bi, a.beh[j], ri, rj, d, a.events))
+      sur_emit_pair(a, i, j);
   }
 }
 
@@ -1146,6 +1145,7 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
   na.pairs = c->pairs;
   na.ctr = c->ctr;
   na.pair_cap = c->pair_cap;
+  na.events = c->p.events;
   const unsigned item_ub = (unsigned)(ncell + (n + 63) / 64 + 1);
   if (detect)
     hipLaunchKernelGGL(k_sur_near<true>, dim3(item_ub), dim3(256), 0, st, na);

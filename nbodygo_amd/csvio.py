@@ -91,7 +91,8 @@ def read_csv(path=None, text=None, body_count=1000, default_collision=L.NB_CB_EL
     flags[suns] |= L.NB_F_SUN
     b = Bodies(a[:, 0], a[:, 1], a[:, 2], a[:, 3], a[:, 4], a[:, 5], a[:, 6], a[:, 7],
                beh=np.array([r[2] for r in rows], np.int32), flags=flags,
-               frag=np.array([r[4] for r in rows], np.float64))
+               frag=np.array([r[4] for r in rows], np.float64),
+               fstep=np.array([r[5] for r in rows], np.float64))
     colors = np.array([r[3] for r in rows], np.int32)
     return b, colors, suns
 
@@ -106,7 +107,7 @@ def write_csv(b: Bodies, path=None, colors=None):
         col = COLOR_NAMES[int(colors[i])] if colors is not None else "random"
         vals = [b.x[i], b.y[i], b.z[i], b.vx[i], b.vy[i], b.vz[i], b.m[i], b.r[i]]
         out.write(",".join(repr(float(v)) for v in vals))
-        out.write(",%s,%s,%s,%r,0\n" % (sun, cb, col, float(b.frag[i])))
+        out.write(",%s,%s,%s,%r,%r\n" % (sun, cb, col, float(b.frag[i]), float(b.fstep[i])))
     s = out.getvalue()
     if path:
         with open(path, "w") as f:

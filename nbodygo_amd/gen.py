@@ -83,7 +83,7 @@ def plummer(n, seed, a=1.0, M=1.0, G=1.0, radius=0.0, mass_cut=0.999, center=(0,
 
 
 def concat(*parts):
-    keys = ("x", "y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest", "frag")
+    keys = ("x", "y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest", "frag", "fstep")
     return Bodies(*(np.concatenate([getattr(p, k) for p in parts]) for k in keys))
 
 

@@ -107,19 +107,28 @@ class BodyCollection:
         adds, self._adds = self._adds, []
         return adds
 
-    def cycle(self, R, keep=None, adds=()):
-        """Mirrors nb_cycle: survivors (keep, default the Exists flags) in order, then the adds
-        ((Bodies, ids) in enqueue order, or Bodies, whose ids are drawn here) newest first with
-        r = R (body_collection.go:253-296).  Returns True when the array changed."""
+    def cycle(self, R, keep=None, adds=(), fragments=0):
+        """Mirrors nb_cycle: survivors (keep, default the Exists flags) in order, then the
+        device's `fragments` fragment bodies (Elastic, radius 1, ids drawn here in creation
+        order), then the adds ((Bodies, ids) in enqueue order, or Bodies, whose ids are drawn
+        here), each group newest first with r = R (body_collection.go:253-296).  Returns True
+        when the array changed."""
         from .gen import concat
         if keep is None:
             keep = (self.b.flags & L.NB_F_EXISTS).astype(bool)
         adds = [a if isinstance(a, tuple) else
                 (a, np.array([next_id() for _ in range(a.n)], np.int64)) for a in adds]
-        changed = not keep.all() or bool(adds)
+        changed = not keep.all() or bool(adds) or fragments > 0
         if not keep.all():
             self.b = _select(self.b, keep)
             self.ids = self.ids[keep]
+        if fragments:
+            z = np.zeros(fragments)
+            fr = Bodies(z, z, z, z, z, z, z, np.full(fragments, 1.0))  # fragcalc.go:81-82
+            fr.rest[:] = R
+            self.b = concat(self.b, fr)
+            self.ids = np.concatenate(
+                [self.ids, np.array([next_id() for _ in range(fragments)], np.int64)[::-1]])
         for a, ids in reversed(adds):
             a = _select(a, np.arange(a.n)[::-1])
             a.rest[:] = R
@@ -130,7 +139,7 @@ class BodyCollection:
 
 
 def _select(b, mask):
-    keys = ("x", "y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest", "frag")
+    keys = ("x", "y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest", "frag", "fstep")
     return Bodies(*(getattr(b, k)[mask].copy() for k in keys))
 
 
@@ -252,11 +261,12 @@ class ComputationRunner:
             gone = False
         self.rqh.add(rq)
         adds = self.bc.take_adds()
-        if gone or adds:  # Cycle on the device, mirrored on the host side
+        pending = self.sim.event_stats()["fragments_pending"] if self.sim.params.events else 0
+        if gone or adds or pending:  # Cycle on the device, mirrored on the host side
             for a, _ in adds:
                 self.sim.add_bodies(a)
             self.sim.cycle(self.R)
-            self.bc.cycle(self.R, ex.astype(bool) if n else None, adds)
+            self.bc.cycle(self.R, ex.astype(bool) if n else None, adds, fragments=pending)
             self._stale = True
             self.stats["compactions"] += 1
         else:

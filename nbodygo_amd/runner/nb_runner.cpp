@@ -323,16 +323,23 @@ std::vector<Body> BodyCollection::TakeAdds() {
   a.swap(adds_);
   return a;
 }
-bool BodyCollection::Cycle(double R, const uint8_t *keep, const std::vector<Body> &adds) {
+bool BodyCollection::Cycle(double R, const uint8_t *keep, const std::vector<Body> &adds,
+                           const std::vector<Body> &frags) {
   std::lock_guard<std::mutex> g(lock_);
   bool changed = false;
   std::vector<Body> a;
-  a.reserve(arr_.size() + adds.size());
+  a.reserve(arr_.size() + frags.size() + adds.size());
   for (size_t i = 0; i < arr_.size(); i++) {
     if (keep ? keep[i] != 0 : arr_[i].exists)
       a.push_back(arr_[i]);
     else
       changed = true;
+  }
+  for (auto it = frags.rbegin(); it != frags.rend(); ++it) {  // newest first, as nb_cycle
+    Body c = *it;
+    c.r = R;
+    a.push_back(c);
+    changed = true;
   }
   for (auto it = adds.rbegin(); it != adds.rend(); ++it) {  // newest first, as nb_cycle
     Body c = *it;
@@ -351,6 +358,7 @@ ComputationRunner::ComputationRunner(int worker_cnt, double time_scaling, bool b
                                      const nb_params &params)
     : worker_cnt_(worker_cnt), time_scaling_(time_scaling), rqh_(rqh), bc_(bc) {
   (void)barnes_hut;
+  events_ = params.events != 0;
   const int rc = nb_create(&params, &ctx_);
   if (rc != NB_OK) {
     err_ = std::string("nb_create failed: ") + nb_last_error(nullptr);
@@ -407,7 +415,7 @@ void ComputationRunner::run() {
 nb_body ComputationRunner::ToDevice(const Body &b) {
   nb_body d{};
   d.x = b.x, d.y = b.y, d.z = b.z, d.vx = b.vx, d.vy = b.vy, d.vz = b.vz;
-  d.m = b.mass, d.r = b.radius, d.rest = b.r, d.frag = b.frag_factor;
+  d.m = b.mass, d.r = b.radius, d.rest = b.r, d.frag = b.frag_factor, d.frag_step = b.frag_step;
   d.beh = b.collision_behavior;
   d.flags = (uint8_t)((b.exists ? NB_F_EXISTS : 0) | (b.fragmenting ? NB_F_FRAGMENTING : 0) |
                       (b.is_sun ? NB_F_SUN : 0) | (b.pinned ? NB_F_PINNED : 0));
@@ -416,7 +424,7 @@ nb_body ComputationRunner::ToDevice(const Body &b) {
 
 void ComputationRunner::FromDevice(const nb_body &d, Body *b) {
   b->x = d.x, b->y = d.y, b->z = d.z, b->vx = d.vx, b->vy = d.vy, b->vz = d.vz;
-  b->mass = d.m, b->radius = d.r, b->r = d.rest, b->frag_factor = d.frag;
+  b->mass = d.m, b->radius = d.r, b->r = d.rest, b->frag_factor = d.frag, b->frag_step = d.frag_step;
   b->collision_behavior = d.beh;
   b->exists = (d.flags & NB_F_EXISTS) != 0;
   b->fragmenting = (d.flags & NB_F_FRAGMENTING) != 0;
@@ -432,12 +440,13 @@ int ComputationRunner::SyncToHost() {
     host_stale_ = false;
     return NB_OK;
   }
-  std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), m(n), r(n), rest(n), frag(n);
+  std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), m(n), r(n), rest(n), frag(n),
+      fstep(n);
   std::vector<int32_t> beh(n);
   std::vector<uint8_t> fl(n);
   const int rc = nb_store_bodies(ctx_, x.data(), y.data(), z.data(), vx.data(), vy.data(),
                                  vz.data(), m.data(), r.data(), beh.data(), fl.data(),
-                                 rest.data(), frag.data());
+                                 rest.data(), frag.data(), fstep.data());
   if (rc != NB_OK) {
     err_ = nb_last_error(ctx_);
     return rc;
@@ -446,6 +455,7 @@ int ComputationRunner::SyncToHost() {
     nb_body d;
     d.x = x[i], d.y = y[i], d.z = z[i], d.vx = vx[i], d.vy = vy[i], d.vz = vz[i];
     d.m = m[i], d.r = r[i], d.rest = rest[i], d.frag = frag[i], d.beh = beh[i], d.flags = fl[i];
+    d.frag_step = fstep[i];
     FromDevice(d, &arr[i]);
   }
   host_stale_ = false;
@@ -457,19 +467,20 @@ int ComputationRunner::pushIfDirty() {
   if (!dirty_) return NB_OK;
   auto &arr = bc_->GetArray();
   const size_t n = arr.size();
-  std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), m(n), r(n), rest(n), frag(n);
+  std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), m(n), r(n), rest(n), frag(n),
+      fstep(n);
   std::vector<int32_t> beh(n);
   std::vector<uint8_t> fl(n);
   for (size_t i = 0; i < n; i++) {
     const nb_body d = ToDevice(arr[i]);
     x[i] = d.x, y[i] = d.y, z[i] = d.z, vx[i] = d.vx, vy[i] = d.vy, vz[i] = d.vz;
-    m[i] = d.m, r[i] = d.r, rest[i] = d.rest, frag[i] = d.frag;
+    m[i] = d.m, r[i] = d.r, rest[i] = d.rest, frag[i] = d.frag, fstep[i] = d.frag_step;
     beh[i] = d.beh;
     fl[i] = d.flags;
   }
-  const int rc = nb_load(ctx_, (int64_t)n, x.data(), y.data(), z.data(), vx.data(), vy.data(),
-                         vz.data(), m.data(), r.data(), beh.data(), fl.data(), rest.data(),
-                         frag.data());
+  int rc = nb_load(ctx_, (int64_t)n, x.data(), y.data(), z.data(), vx.data(), vy.data(),
+                   vz.data(), m.data(), r.data(), beh.data(), fl.data(), rest.data(), frag.data());
+  if (rc == NB_OK && events_) rc = nb_set_frag_steps(ctx_, fstep.data());
   if (rc != NB_OK) {
     err_ = nb_last_error(ctx_);
     return rc;
@@ -662,32 +673,44 @@ int ComputationRunner::cycle() {
   rqh_->Add(std::move(*rq));
   // Cycle: compaction and adds on the device, the host collection compacted alongside
   std::vector<Body> adds = bc_->TakeAdds();
-  if (gone || !adds.empty()) {
+  nb_event_stats evs{};
+  if (events_) nb_get_event_stats(ctx_, &evs);
+  if (gone || !adds.empty() || evs.fragments_pending > 0) {
     const size_t k = adds.size();
-    std::vector<double> ax(k), ay(k), az(k), avx(k), avy(k), avz(k), am(k), ar(k), af(k);
+    std::vector<double> ax(k), ay(k), az(k), avx(k), avy(k), avz(k), am(k), ar(k), af(k), afs(k);
     std::vector<int32_t> abeh(k);
     std::vector<uint8_t> afl(k);
     for (size_t i = 0; i < k; i++) {
       const nb_body d = ToDevice(adds[i]);
       ax[i] = d.x, ay[i] = d.y, az[i] = d.z, avx[i] = d.vx, avy[i] = d.vy, avz[i] = d.vz;
-      am[i] = d.m, ar[i] = d.r, af[i] = d.frag, abeh[i] = d.beh, afl[i] = d.flags;
+      am[i] = d.m, ar[i] = d.r, af[i] = d.frag, afs[i] = d.frag_step, abeh[i] = d.beh;
+      afl[i] = d.flags;
     }
     int64_t n_new = 0;
     rc = nb_add_bodies(ctx_, (int64_t)k, ax.data(), ay.data(), az.data(), avx.data(), avy.data(),
-                       avz.data(), am.data(), ar.data(), abeh.data(), afl.data(), af.data());
+                       avz.data(), am.data(), ar.data(), abeh.data(), afl.data(), af.data(),
+                       afs.data());
     if (rc == NB_OK) rc = nb_cycle(ctx_, R_, &n_new);
     if (rc != NB_OK) {
       err_ = nb_last_error(ctx_);
       return rc;
     }
-    bc_->Cycle(R_, n ? ex_.data() : nullptr, adds);
+    // the device's fragments (fragment(), fragcalc.go:97-117) come before the host's adds:
+    // Elastic bodies of radius 1 with fresh ids, newest first
+    std::vector<Body> frags((size_t)evs.fragments_pending);
+    for (auto &f : frags) {
+      f.id = NextId();
+      f.radius = kFragmentRadius;
+      f.collision_behavior = NB_CB_ELASTIC;
+    }
+    bc_->Cycle(R_, n ? ex_.data() : nullptr, adds, frags);
     if ((size_t)n_new != bc_->GetArray().size()) {
       err_ = "device and host collections disagree on the body count after Cycle";
       return NB_E_STATE;
     }
     host_stale_ = true;
   } else {
-    bc_->Cycle(R_, nullptr, adds);
+    bc_->Cycle(R_, nullptr, adds, {});
   }
   std::lock_guard<std::mutex> g(stats_m_);
   st_.computations++;

@@ -55,6 +55,7 @@ struct Body {
 };
 
 int64_t NextId();  // body.go:316-329
+constexpr double kFragmentRadius = 1.0;  // fragcalc.go:81-82: Pow(x, 1/3) with 1/3 == 0
 
 // renderable.go:9-40
 struct Renderable {
@@ -116,9 +117,10 @@ class BodyCollection {
   }
   std::vector<Body> TakeAdds();
   // The host half of nb_cycle: keeps the bodies whose keep flag is set (nullptr: the exists
-  // field), then appends adds newest first, each with r = R.  Returns true when the array
-  // changed.
-  bool Cycle(double R, const uint8_t *keep, const std::vector<Body> &adds);
+  // field), then appends the device's fragments and then adds, each newest first with r = R.
+  // Returns true when the array changed.
+  bool Cycle(double R, const uint8_t *keep, const std::vector<Body> &adds,
+             const std::vector<Body> &frags);
   int cycle() const { return cycle_; }
 
  private:
@@ -214,6 +216,7 @@ class ComputationRunner {
   nb_ctx *ctx_ = nullptr;
   std::string err_;
   bool dirty_ = true;       // nothing loaded on the device yet
+  bool events_ = false;     // nb_params.events: subsume and fragmentation on the device
   bool host_stale_ = false; // device advanced since the last pull
   int max_iteration_ = 0;
   std::atomic<bool> running_{false}, stop_{false};

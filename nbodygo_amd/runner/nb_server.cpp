@@ -18,6 +18,7 @@
 //   --no-cutoff             softened form without the d > Ri+Rj gate
 //   --no-collisions         skip detection and resolution
 //   --fp64                  FP64 force path (bit-identical to the reference's arithmetic)
+//   --events                subsume and fragmentation events on the device (nb_params.events)
 //   --remove N@K            before cycle K, ask the runner to remove N bodies (RemoveBodies)
 //   --add PATH@K            before cycle K, enqueue every body of the CSV at PATH (AddBody);
 //                           may repeat
@@ -41,7 +42,7 @@ static void usage() {
                "usage: nb_server -f bodies.csv [-b N] [-c behaviour] [-m dt] [-u ms] "
                "[--iterations N]\n"
                "                 [--restitution R] [--G G] [--eps EPS] [--no-cutoff] "
-               "[--no-collisions] [--fp64]\n"
+               "[--no-collisions] [--fp64] [--events]\n"
                "                 [--remove N@K] [--add PATH@K] [--mod ID@K:M,...] [--get ID@K]\n"
                "                 [--out final.csv] [--parse-only]\n");
 }
@@ -96,6 +97,7 @@ int main(int argc, char **argv) {
     else if (a == "--no-cutoff") p.overlap_cutoff = 0;
     else if (a == "--no-collisions") p.collisions = 0;
     else if (a == "--fp64") p.precision = NB_PREC_FP64;
+    else if (a == "--events") p.events = 1;
     else if (a == "--remove") {
       const std::string s = val();
       const size_t at = s.find('@');

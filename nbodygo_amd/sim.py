@@ -27,6 +27,7 @@ class Bodies:
     flags: np.ndarray = None
     rest: np.ndarray = None
     frag: np.ndarray = None
+    fstep: np.ndarray = None  # Body.FragStep (fragcalc.go:72)
 
     def __post_init__(self):
         for k in ("x", "y", "z", "vx", "vy", "vz", "m", "r"):
@@ -44,7 +45,10 @@ class Bodies:
         if self.frag is None:
             self.frag = np.zeros(n, np.float64)
         self.frag = np.ascontiguousarray(self.frag, dtype=np.float64)
-        for k in ("y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest", "frag"):
+        if self.fstep is None:
+            self.fstep = np.zeros(n, np.float64)
+        self.fstep = np.ascontiguousarray(self.fstep, dtype=np.float64)
+        for k in ("y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest", "frag", "fstep"):
             if getattr(self, k).shape != (n,):
                 raise ValueError("Bodies.%s has shape %s, expected (%d,)" %
                                  (k, getattr(self, k).shape, n))
@@ -56,7 +60,7 @@ class Bodies:
     def copy(self):
         return Bodies(*(getattr(self, k).copy() for k in
                         ("x", "y", "z", "vx", "vy", "vz", "m", "r", "beh", "flags", "rest",
-                         "frag")))
+                         "frag", "fstep")))
 
     def pos(self):
         return np.stack([self.x, self.y, self.z], axis=1)
@@ -80,7 +84,7 @@ class NBodyGPU:
 
     def __init__(self, G=6.673e-11, eps=0.0, overlap_cutoff=True, collisions=True,
                  precision="fp32", device=0, pair_capacity=0, ipl=0, tile_j=0, jsplit=0,
-                 block=0, variant=0):
+                 block=0, variant=0, events=False):
         p = L.NbParams()
         check(lib.nb_params_init(C.byref(p)))
         p.G = float(G)
@@ -91,6 +95,7 @@ class NBodyGPU:
         p.device = int(device)
         p.pair_capacity = int(pair_capacity)
         p.ipl, p.tile_j, p.jsplit, p.block, p.variant = ipl, tile_j, jsplit, block, variant
+        p.events = int(bool(events))
         self.params = p
         self._ctx = C.c_void_p()
         check(lib.nb_create(C.byref(p), C.byref(self._ctx)), None)
@@ -122,6 +127,8 @@ class NBodyGPU:
                                 b.beh.ctypes.data_as(L._i32p), b.flags.ctypes.data_as(L._u8p),
                                 _dp(b.rest), _dp(b.frag)))
         self.n = b.n
+        if self.params.events:
+            self._check(lib.nb_set_frag_steps(self._ctx, _dp(b.fstep)))
         return self
 
     def step(self, dt, R=1.0, nsteps=1, allow_overflow=False):
@@ -159,10 +166,12 @@ class NBodyGPU:
         flags = np.empty(n, np.uint8)
         rest = np.empty(n, np.float64)
         frag = np.empty(n, np.float64)
+        fstep = np.empty(n, np.float64)
         self._check(lib.nb_store_bodies(self._ctx, *(_dp(a) for a in f),
                                         beh.ctypes.data_as(L._i32p),
-                                        flags.ctypes.data_as(L._u8p), _dp(rest), _dp(frag)))
-        return Bodies(*f, beh=beh, flags=flags, rest=rest, frag=frag)
+                                        flags.ctypes.data_as(L._u8p), _dp(rest), _dp(frag),
+                                        _dp(fstep)))
+        return Bodies(*f, beh=beh, flags=flags, rest=rest, frag=frag, fstep=fstep)
 
     # -- Barnes-Hut octree (cmd/body/octree.go with Build wired in) ---------------------------
     def bh_enable(self, theta=0.3, softening=1e-4, max_depth=21, group=0):
@@ -191,7 +200,8 @@ class NBodyGPU:
         self._check(lib.nb_add_bodies(self._ctx, b.n, _dp(b.x), _dp(b.y), _dp(b.z), _dp(b.vx),
                                       _dp(b.vy), _dp(b.vz), _dp(b.m), _dp(b.r),
                                       b.beh.ctypes.data_as(L._i32p),
-                                      b.flags.ctypes.data_as(L._u8p), _dp(b.frag)))
+                                      b.flags.ctypes.data_as(L._u8p), _dp(b.frag),
+                                      _dp(b.fstep)))
 
     def cycle(self, R=1.0):
         """Compaction of removed bodies, then the queued adds with restitution R
@@ -206,6 +216,11 @@ class NBodyGPU:
         k = C.c_int64()
         self._check(lib.nb_remove_bodies(self._ctx, int(count), C.byref(k)))
         return k.value
+
+    def event_stats(self):
+        st = L.NbEventStats()
+        self._check(lib.nb_get_event_stats(self._ctx, C.byref(st)))
+        return st.as_dict()
 
     def get_body(self, i):
         out = L.NbBody()

@@ -40,7 +40,8 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     structs = {"nb_params": L.NbParams, "nb_stats": L.NbStats, "nb_diag": L.NbDiag,
                "nb_sur_params": L.NbSurParams, "nb_sur_stats": L.NbSurStats,
-               "nb_body": L.NbBody, "nb_bh_params": L.NbBhParams, "nb_bh_stats": L.NbBhStats}
+               "nb_body": L.NbBody, "nb_bh_params": L.NbBhParams, "nb_bh_stats": L.NbBhStats,
+               "nb_event_stats": L.NbEventStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nbody_gpu.h"',
              "int main(void){"]
     for cname, cls in structs.items():
@@ -111,12 +112,15 @@ def test_surrogate_params_defaults_and_null_handling():
 def test_lifecycle_null_handling():
     b = L.NbBody()
     n = C.c_int64()
-    assert L.lib.nb_add_bodies(None, 0, *([None] * 11)) == L.NB_E_INVALID
+    assert L.lib.nb_add_bodies(None, 0, *([None] * 12)) == L.NB_E_INVALID
     assert L.lib.nb_cycle(None, 1.0, C.byref(n)) == L.NB_E_INVALID
     assert L.lib.nb_remove_bodies(None, 1, C.byref(n)) == L.NB_E_INVALID
     assert L.lib.nb_get_body(None, 0, C.byref(b)) == L.NB_E_INVALID
     assert L.lib.nb_set_body(None, 0, C.byref(b)) == L.NB_E_INVALID
-    assert L.lib.nb_store_bodies(None, *([None] * 12)) == L.NB_E_INVALID
+    assert L.lib.nb_store_bodies(None, *([None] * 13)) == L.NB_E_INVALID
+    assert L.lib.nb_set_frag_steps(None, None) == L.NB_E_INVALID
+    ev = L.NbEventStats()
+    assert L.lib.nb_get_event_stats(None, C.byref(ev)) == L.NB_E_INVALID
 
 
 def test_barnes_hut_params_defaults_and_null_handling():
