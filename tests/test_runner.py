@@ -267,3 +267,37 @@ def test_python_runner_skips_cycles_when_the_consumer_lags():
     finally:
         run.close()
     assert run.stats["computations"] == 2 and run.stats["skipped_no_queue"] == 3
+
+
+@pytest.mark.gpu
+def test_cpp_driver_with_events_matches_oracle(tmp_path):
+    """nb_server --events: subsumes and fragmentation through the C++ runner's cycles (the
+    device queue appended by nb_cycle, the host collection compacted alongside)."""
+    from oracle import events as EV
+    b = gen.uniform_sphere(600, seed=19, radius_k=0.25)
+    rng = np.random.default_rng(19)
+    u = rng.random(b.n)
+    b.beh[u < 0.3] = L.NB_CB_SUBSUME
+    b.beh[(u >= 0.3) & (u < 0.5)] = L.NB_CB_FRAGMENT
+    b.frag[:] = rng.uniform(0.0, 2.0, b.n)
+    b.fstep[:] = 3.0
+    src = tmp_path / "ev.csv"
+    csvio.write_csv(b, str(src))
+    b, _, _ = csvio.read_csv(str(src))
+    out = tmp_path / "final.csv"
+    cmd = [SERVER, "-f", str(src), "-b", "600", "-m", "1e-3", "--iterations", "8", "--fp64",
+           "--events", "--G", "1.0", "--eps", "0.01", "--no-cutoff", "--out", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    sim = EV.EventSim(b, 1.0, eps=0.01, cutoff=False)
+    for _ in range(8):
+        sim.step(1e-3, 1.0)
+        sim.cycle(1.0)
+    assert sim.stats["subsumed"] > 10 and sim.stats["emitted"] > 10
+    got, _, _ = csvio.read_csv(str(out), body_count=10 ** 6)
+    live = (sim.flags & EV.EXISTS) != 0
+    exp = np.stack([sim.a[k][live] for k in ("x", "y", "z", "vx", "vy", "vz")], 1)
+    assert got.n == exp.shape[0]
+    np.testing.assert_allclose(np.stack([got.x, got.y, got.z, got.vx, got.vy, got.vz], 1), exp,
+                               rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(got.m, sim.a["m"][live], rtol=1e-12)
