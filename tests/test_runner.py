@@ -301,3 +301,40 @@ def test_cpp_driver_with_events_matches_oracle(tmp_path):
     np.testing.assert_allclose(np.stack([got.x, got.y, got.z, got.vx, got.vy, got.vz], 1), exp,
                                rtol=1e-10, atol=1e-9)
     np.testing.assert_allclose(got.m, sim.a["m"][live], rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_python_runner_with_events_matches_oracle():
+    from oracle import events as EV
+    b = gen.uniform_sphere(500, seed=20, radius_k=0.25)
+    rng = np.random.default_rng(20)
+    u = rng.random(b.n)
+    b.beh[u < 0.3] = L.NB_CB_SUBSUME
+    b.beh[(u >= 0.3) & (u < 0.5)] = L.NB_CB_FRAGMENT
+    b.frag[:] = rng.uniform(0.0, 2.0, b.n)
+    b.fstep[:] = 3.0
+    rqh = R.ResultQueueHolder(4)
+    bc = R.BodyCollection(b)
+    run = R.ComputationRunner(1, 1e-3, rqh, bc, precision="fp64", collisions=True, events=True,
+                              G=1.0, eps=0.01, overlap_cutoff=False)
+    sim = EV.EventSim(b, 1.0, eps=0.01, cutoff=False)
+    try:
+        for _ in range(6):
+            n = bc.count()
+            run.run_one_computation()
+            q = rqh.next()
+            assert q is not None and len(q.queue) == n
+            sim.step(1e-3, 1.0)
+            sim.cycle(1.0)
+            assert bc.count() == sim.n
+        run.sync_to_host()
+    finally:
+        run.close()
+    assert sim.stats["subsumed"] > 10 and sim.stats["emitted"] > 10
+    assert len(set(bc.ids.tolist())) == bc.count()  # every body, fragments too, has its own id
+    np.testing.assert_array_equal(bc.b.flags, sim.flags)
+    np.testing.assert_allclose(bc.b.m, sim.a["m"], rtol=1e-12)
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_allclose(getattr(bc.b, k), sim.a[k], rtol=1e-10, atol=1e-9)
+    np.testing.assert_array_equal(bc.b.r, sim.a["r"])  # fragments have radius 1
+    assert (bc.b.r == 1.0).sum() > 10
