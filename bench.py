@@ -49,6 +49,11 @@ WORKLOADS = {
                                          "collisions off: the reference's Barnes-Hut octree "
                                          "(theta 0.3, softening 1e-4, Build wired in) in FP64 "
                                          "against the FP64 direct sum with the same softening"),
+    "lifecycle1m": dict(n=1048576, desc="Plummer sphere N=1,048,576, FP32 path: one delete "
+                                       "request of 1 % of the bodies (processDeletes' walk), "
+                                       "1,000 queued adds and Cycle, on the device, against "
+                                       "the same through the host (read back, compact, "
+                                       "reload)"),
     "batch512": dict(n=0, desc="512 independent simulations, N uniform in [256, 8192] (seeded), "
                                "uniform-sphere clusters with lognormal masses, eps=0.01, "
                                "dt=1e-3, FP32 force + Kahan, FP64 update, one hipGraph launch "
@@ -332,6 +337,44 @@ def run_barneshut(a, w):
     }))
 
 
+def run_lifecycle(a, w):
+    """Row (f)1: the collection bookkeeping on the device (nb_remove_bodies + nb_add_bodies +
+    nb_cycle) against a host round trip (nb_store_bodies, numpy compaction, nb_load)."""
+    from nbodygo_amd import NBodyGPU, gen
+    n = w["n"]
+    b = gen.plummer(n, seed=2024)
+    add = gen.plummer(1000, seed=7)
+    reps = max(1, a.steps // 10)
+    g = NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=False)
+    dev, host = [], []
+    for _ in range(reps):
+        g.load(b)
+        g.step(1e-3, 1.0, 1)
+        t0 = time.perf_counter()
+        g.remove_bodies(n // 100)
+        g.add_bodies(add)
+        n_dev = g.cycle(1.0)
+        dev.append(time.perf_counter() - t0)
+        g.load(b)
+        g.step(1e-3, 1.0, 1)
+        t0 = time.perf_counter()
+        s = g.store()
+        keep = np.ones(s.n, bool)
+        keep[::100] = False
+        from nbodygo_amd.gen import concat
+        from nbodygo_amd.runner import _select
+        g.load(concat(_select(s, keep), add))
+        host.append(time.perf_counter() - t0)
+    g.close()
+    dms, hms = 1e3 * float(np.median(dev)), 1e3 * float(np.median(host))
+    print(json.dumps({
+        "metric": METRIC, "value": 1e3 / dms, "unit": "cycles/s (delete 1 % + 1,000 adds + Cycle)",
+        "n_gpus": 1, "steps": reps, "warmup": 0, "ms_per_step": dms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64 state", "data": "synthetic",
+        "config": {"workload": w["desc"], "n": n, "bodies_after": n_dev},
+        "device_ms": dms, "host_round_trip_ms": hms, "speedup": hms / dms}))
+
+
 def cpu_baseline(b, seconds):
     """The reference's worker-pool CPU algorithm (oracle restatement) on a bounded sample: the
     first T targets of the same body set against all N sources, one Compute phase."""
@@ -367,6 +410,8 @@ def main():
         return run_collide(a, w, shape_kw(a.shape))
     if a.workload == "barneshut262k":
         return run_barneshut(a, w)
+    if a.workload == "lifecycle1m":
+        return run_lifecycle(a, w)
     if a.workload == "surrogate262k":
         sur = surrogate_leg(a, shape_kw(a.shape))
         n = sur["n"]

@@ -44,7 +44,26 @@ static hipError_t dalloc(T **p, size_t count) {
   return hipMalloc((void **)p, count * sizeof(T));
 }
 
+static void free_spare(Ctx *c) {
+  auto &S = c->spare;
+  for (auto *&q : S.a)
+    if (q) (void)hipFree(q), q = nullptr;
+  void *ps[] = {S.beh, S.flags, S.collided, S.fstep, S.fst, c->cyc_keep, c->cyc_pos, c->cyc_tmp};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  S.beh = nullptr;
+  S.flags = S.collided = nullptr;
+  S.fstep = nullptr;
+  S.fst = nullptr;
+  S.cap = 0;
+  c->cyc_keep = c->cyc_pos = nullptr;
+  c->cyc_tmp = nullptr;
+  c->cyc_tmp_bytes = 0;
+  c->cyc_cap = 0;
+}
+
 static void free_all(Ctx *c) {
+  free_spare(c);
   void *ptrs[] = {c->x,    c->y,      c->z,     c->vx,   c->vy,        c->vz,   c->m,
                   c->r,    c->rest,   c->frag,  c->beh,  c->flags,     c->collided,
                   c->js,   c->rmax_tile, c->acc, c->fcnt, c->flist, c->recs, c->pairs, c->ctr,
@@ -831,28 +850,25 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
   NB_HIP(c, hipSetDevice(c->device));
   hipStream_t st = c->stream;
   const int64_t n = c->n, nadd = c->adds.count();
-  int32_t *keep = nullptr, *pos = nullptr;
-  void *tmp = nullptr;
   int64_t nk = 0;
-  // one owner for the scratch buffers of this call
-  struct Scratch {
-    int32_t **a, **b;
-    void **t;
-    ~Scratch() {
-      if (*a) (void)hipFree(*a);
-      if (*b) (void)hipFree(*b);
-      if (*t) (void)hipFree(*t);
-    }
-  } scratch{&keep, &pos, &tmp};
   if (n > 0) {
-    NB_HIP(c, hipMalloc((void **)&keep, (size_t)n * sizeof(int32_t)));
-    NB_HIP(c, hipMalloc((void **)&pos, (size_t)n * sizeof(int32_t)));
+    if (c->cyc_cap < n) {
+      NB_HIP(c, dalloc(&c->cyc_keep, (size_t)c->cap));
+      NB_HIP(c, dalloc(&c->cyc_pos, (size_t)c->cap));
+      c->cyc_cap = c->cap;
+    }
+    int32_t *keep = c->cyc_keep, *pos = c->cyc_pos;
     NB_HIP(c, launch_keep_flags(c->flags, (int)n, keep, st));
     size_t tb = 0;
     NB_HIP(c, rocprim::exclusive_scan(nullptr, tb, keep, pos, 0, (size_t)n,
                                       rocprim::plus<int32_t>(), st));
-    NB_HIP(c, hipMalloc(&tmp, tb ? tb : 1));
-    NB_HIP(c, rocprim::exclusive_scan(tmp, tb, keep, pos, 0, (size_t)n,
+    if (tb > c->cyc_tmp_bytes) {
+      if (c->cyc_tmp) (void)hipFree(c->cyc_tmp);
+      c->cyc_tmp = nullptr;
+      NB_HIP(c, hipMalloc(&c->cyc_tmp, tb));
+      c->cyc_tmp_bytes = tb;
+    }
+    NB_HIP(c, rocprim::exclusive_scan(c->cyc_tmp, tb, keep, pos, 0, (size_t)n,
                                       rocprim::plus<int32_t>(), st));
     int32_t lp = 0, lk = 0;
     NB_HIP(c, hipMemcpyAsync(&lp, pos + n - 1, 4, hipMemcpyDeviceToHost, st));
@@ -875,32 +891,39 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
   if (n_new > (int64_t)1 << 30) return fail(c, NB_E_INVALID, "nb_cycle: too many bodies");
   const int64_t npad_new = std::max<int64_t>(round_up(n_new, kPad), kPad);
   const int64_t cap_new = std::max<int64_t>(c->cap, npad_new);
-  double *d[10] = {};
-  int32_t *beh = nullptr;
-  uint8_t *fl = nullptr, *col = nullptr;
-  double *fs = nullptr;
-  FragState *fst = nullptr;
-  auto release = [&]() {
-    for (auto *q : d)
-      if (q) (void)hipFree(q);
-    if (beh) (void)hipFree(beh);
-    if (fl) (void)hipFree(fl);
-    if (col) (void)hipFree(col);
-    if (fs) (void)hipFree(fs);
-    if (fst) (void)hipFree(fst);
-  };
-  bool ok = true;
-  for (int q = 0; q < 10 && ok; q++) ok = hipMalloc((void **)&d[q], (size_t)cap_new * 8) == hipSuccess;
-  ok = ok && hipMalloc((void **)&beh, (size_t)cap_new * 4) == hipSuccess &&
-       hipMalloc((void **)&fl, (size_t)cap_new) == hipSuccess &&
-       hipMalloc((void **)&col, (size_t)cap_new) == hipSuccess;
-  if (ok && c->p.events)
-    ok = hipMalloc((void **)&fs, (size_t)cap_new * 8) == hipSuccess &&
-         hipMalloc((void **)&fst, (size_t)cap_new * sizeof(FragState)) == hipSuccess;
-  if (!ok) {
-    release();
-    return fail(c, NB_E_NOMEM, "nb_cycle: device allocation failed");
+  auto &S = c->spare;
+  if (S.cap < cap_new || (c->p.events && !S.fst)) {  // (re)allocate the second set
+    for (auto *&q : S.a)
+      if (q) (void)hipFree(q), q = nullptr;
+    void *ps[] = {S.beh, S.flags, S.collided, S.fstep, S.fst};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    S.beh = nullptr;
+    S.flags = S.collided = nullptr;
+    S.fstep = nullptr;
+    S.fst = nullptr;
+    S.cap = 0;
+    bool ok = true;
+    for (int q = 0; q < 10 && ok; q++)
+      ok = hipMalloc((void **)&S.a[q], (size_t)cap_new * 8) == hipSuccess;
+    ok = ok && hipMalloc((void **)&S.beh, (size_t)cap_new * 4) == hipSuccess &&
+         hipMalloc((void **)&S.flags, (size_t)cap_new) == hipSuccess &&
+         hipMalloc((void **)&S.collided, (size_t)cap_new) == hipSuccess;
+    if (ok && c->p.events)
+      ok = hipMalloc((void **)&S.fstep, (size_t)cap_new * 8) == hipSuccess &&
+           hipMalloc((void **)&S.fst, (size_t)cap_new * sizeof(FragState)) == hipSuccess;
+    if (!ok) {
+      free_spare(c);
+      return fail(c, NB_E_NOMEM, "nb_cycle: device allocation failed");
+    }
+    S.cap = cap_new;
   }
+  double *d[10];
+  for (int q = 0; q < 10; q++) d[q] = S.a[q];
+  int32_t *beh = S.beh;
+  uint8_t *fl = S.flags, *col = S.collided;
+  double *fs = c->p.events ? S.fstep : nullptr;
+  FragState *fst = c->p.events ? S.fst : nullptr;
   hipError_t e = hipSuccess;
   for (int q = 0; q < 10 && e == hipSuccess; q++)
     e = hipMemsetAsync(d[q], 0, (size_t)cap_new * 8, st);
@@ -919,7 +942,7 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
   ca.beh_dst = beh;
   ca.flags_src = c->flags;
   ca.flags_dst = fl;
-  ca.pos = pos;
+  ca.pos = c->cyc_pos;
   ca.fstep_src = c->fstep;
   ca.fstep_dst = fs;
   ca.fst_src = c->fst;
@@ -947,31 +970,22 @@ int nb_cycle(nb_ctx *c, double R, int64_t *n_out) {
   if (e == hipSuccess && nadd && fs)
     e = hipMemcpyAsync(fs + at, A.fstep.data(), (size_t)nadd * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (e != hipSuccess) {
-    release();
-    NB_HIP(c, e);
-  }
-  // swap in the new arrays
+  NB_HIP(c, e);
+  // swap the sets: the compacted arrays go live, the old ones become the second set
+  const int64_t live_cap = S.cap;  // >= cap_new >= c->cap
   double **cur[10] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->m, &c->r, &c->rest,
                       &c->frag};
-  for (int q = 0; q < 10; q++) {
-    (void)hipFree(*cur[q]);
-    *cur[q] = d[q];
-  }
-  (void)hipFree(c->beh);
-  (void)hipFree(c->flags);
-  (void)hipFree(c->collided);
-  c->beh = beh;
-  c->flags = fl;
-  c->collided = col;
+  for (int q = 0; q < 10; q++) std::swap(*cur[q], S.a[q]);
+  std::swap(c->beh, S.beh);
+  std::swap(c->flags, S.flags);
+  std::swap(c->collided, S.collided);
   if (c->p.events) {
-    if (c->fstep) (void)hipFree(c->fstep);
-    if (c->fst) (void)hipFree(c->fst);
-    c->fstep = fs;
-    c->fst = fst;
+    std::swap(c->fstep, S.fstep);
+    std::swap(c->fst, S.fst);
   }
-  if (cap_new > c->cap) {
-    c->cap = cap_new;
+  S.cap = c->cap;  // the old live set's capacity
+  if (live_cap > c->cap) {
+    c->cap = live_cap;
     NB_HIP(c, dalloc(&c->js, c->cap));
     c->diag_len = c->cap * 8 + 64;
     NB_HIP(c, dalloc(&c->diag, c->diag_len));

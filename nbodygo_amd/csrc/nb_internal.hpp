@@ -145,6 +145,21 @@ struct Ctx {
   void *ev_scan_tmp = nullptr;
   size_t ev_scan_bytes = 0;
 
+  // nb_cycle compacts into a second set of per-body arrays and swaps it with the live one, so a
+  // cycle allocates nothing once both sets have the capacity
+  struct Spare {
+    double *a[10] = {};  // x y z vx vy vz m r rest frag
+    int32_t *beh = nullptr;
+    uint8_t *flags = nullptr, *collided = nullptr;
+    double *fstep = nullptr;
+    FragState *fst = nullptr;
+    int64_t cap = 0;
+  } spare;
+  int32_t *cyc_keep = nullptr, *cyc_pos = nullptr;  // keep flags and their prefix sum
+  void *cyc_tmp = nullptr;
+  size_t cyc_tmp_bytes = 0;
+  int64_t cyc_cap = 0;
+
   // bodies queued by nb_add_bodies, appended by the next nb_cycle
   struct Adds {
     std::vector<double> a[10];  // x y z vx vy vz m r rest frag
