@@ -90,17 +90,18 @@ static bool drop_needed(const Ctx *c) { return c->p.overlap_cutoff || c->p.eps =
 
 static int tiles_of(const Ctx *c) { return (int)(c->npad / c->shape.tile_j); }
 
-// Default launch shape: enough workgroups for 8 per CU on 256 CUs.
+// Default launch shape: the one nb_autotune picks at 65,536 and 262,144 bodies (LDS-staged
+// sources, one target per lane, 512-source tiles), with enough source slices for about 16
+// workgroups per CU on 256 CUs.  npad is a multiple of 1024, so every field divides it.
 static ForceShape default_shape(int64_t npad) {
   ForceShape s;
-  s.ipl = 2;
+  s.ipl = 1;
   s.block = 256;
-  s.tile_j = 256;
-  s.variant = 1;
-  while (npad % (s.block * s.ipl)) s.ipl >>= 1;
+  s.tile_j = 512;
+  s.variant = 2;
   int64_t bx = npad / (s.block * s.ipl);
   int64_t tiles = npad / s.tile_j;
-  int64_t want = (2048 + bx - 1) / bx;
+  int64_t want = (4096 + bx - 1) / bx;
   s.jsplit = (int)std::max<int64_t>(1, std::min<int64_t>({want, (int64_t)kMaxSplit, tiles}));
   return s;
 }
