@@ -19,11 +19,14 @@
 //           (The reference folds bodies into each node one at a time in array order, so a
 //           node's centre of mass can differ from it in the last bits; leaves are identical.)
 //           A bucket folds its bodies in array order as updateMass does.
-// Force (accumForce, octree.go:276-309): one lane per body, bodies in key order.  A wave walks
-// the tree depth first with a shared LDS stack of (node, lane mask): each lane accepts or opens a
-// node by its own test (2*half/dist) < theta, children are pushed in reverse octant order, so
-// every body's sum has the reference's terms in the reference's order.  Suns sum directly over
-// all bodies in array order (CalcForce, octree.go:252-267).
+// Links, after the masses: each live child slot gets the slot to visit when the walk opens it
+// (its first live child) and the one to visit when it is accepted or done (its next live
+// sibling, else its parent's), so a depth-first walk needs no stack.
+// Force (accumForce, octree.go:276-309): one lane per body, bodies in key order.  Each lane walks
+// the tree on its own by those links: it accepts or opens a node by its own test
+// (2*half/dist) < theta, and children come in octant order, so every body's sum has the
+// reference's terms in the reference's order.  Suns sum directly over all bodies in array order
+// (CalcForce, octree.go:252-267).
 // Collisions: a second walk opens every node whose box lies within R_i + R_max of the body and
 // applies the reference's test at the leaves (Collided, body.go:192-208); pairs go to the same
 // canonical list as the direct path.
@@ -52,15 +55,19 @@ struct BhCounters {
   uint32_t buckets;
   uint32_t stack_overflow;
   uint32_t node_overflow;
+  int32_t root_first;                // the root's first live child slot (-1: none)
   unsigned long long interactions;   // accepted nodes + leaf and direct terms, all targets
 };
 
 // A child as the force walk needs it, eight per node: centre of mass and mass (a leaf's from
-// updateMass from zero), half (-1 marks a bucket), and the child's code: -1 empty, <= -2 the
-// leaf of sorted body -2 - code, else a node.
-struct BhSlot {
+// updateMass from zero), half (-1 marks a bucket), the child's code (-1 empty, <= -2 the leaf of
+// sorted body -2 - code, else a node), and the walk's links: `first` (slot of its first live
+// child, for a node that is opened) and `skip` (slot visited next once this subtree is done;
+// -1 ends the walk).  A slot is live when its code is not -1 and its mass is > 0.  One slot is
+// one 64-byte line.
+struct alignas(64) BhSlot {
   double x, y, z, m, half;
-  int32_t code, pad;
+  int32_t code, first, skip, pad;
 };
 
 struct Bh {
@@ -74,6 +81,7 @@ struct Bh {
   double4 *nbox = nullptr, *nmc = nullptr;
   int32_t *nchild = nullptr;
   BhSlot *slots = nullptr;
+  int32_t *pslot = nullptr;  // each node's slot in its parent
   int2 *nrange = nullptr;
   double *bpart = nullptr;
   void *sort_tmp = nullptr;
@@ -373,6 +381,7 @@ __global__ __launch_bounds__(256) void k_bh_mass(BhMassArgs a) {
       const int32_t code = ch[c];
       BhSlot o;
       o.code = code;
+      o.first = o.skip = -1;
       o.pad = 0;
       if (code == -1) {
         o.x = o.y = o.z = o.m = o.half = 0.0;
@@ -404,6 +413,65 @@ __global__ __launch_bounds__(256) void k_bh_mass(BhMassArgs a) {
   }
 }
 
+// ---- links -------------------------------------------------------------------------------------
+__device__ __forceinline__ int bh_nodes(const BhCounters *cnt, int D, int64_t node_cap) {
+  return (int)min((int64_t)cnt->level[D] + cnt->lcount[D], node_cap);
+}
+
+__device__ __forceinline__ bool bh_live(const BhSlot &sl) { return sl.code != -1 && sl.m > 0; }
+
+// Each node's slot in its parent (the root has none).
+__global__ __launch_bounds__(256) void k_bh_parent(const BhCounters *cnt, const int32_t *nchild,
+                                                   int D, int64_t node_cap, int32_t *pslot) {
+  const int nodes = bh_nodes(cnt, D, node_cap);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nodes; k += gridDim.x * blockDim.x) {
+    const int32_t *ch = nchild + (size_t)k * 8;
+    if (ch[0] == kBucket) continue;
+    for (int c = 0; c < 8; c++)
+      if (ch[c] >= 0) pslot[ch[c]] = k * 8 + c;
+  }
+}
+
+// first and skip of every live slot; the root's first live child goes to the counters.
+__global__ __launch_bounds__(256) void k_bh_links(BhCounters *cnt, const int32_t *nchild,
+                                                  const int32_t *pslot, int D, int64_t node_cap,
+                                                  BhSlot *slots) {
+  const int nodes = bh_nodes(cnt, D, node_cap);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nodes; k += gridDim.x * blockDim.x) {
+    if (nchild[(size_t)k * 8] == kBucket) continue;
+    BhSlot *sl = slots + (size_t)k * 8;
+    if (k == 0) {
+      int f = -1;
+      for (int c = 0; c < 8 && f < 0; c++)
+        if (bh_live(sl[c])) f = c;
+      cnt->root_first = f;
+    }
+    for (int c = 0; c < 8; c++) {
+      if (!bh_live(sl[c])) continue;
+      int first = -1;
+      const int32_t code = sl[c].code;
+      if (code >= 0 && sl[c].half >= 0) {
+        const BhSlot *cs = slots + (size_t)code * 8;
+        for (int c2 = 0; c2 < 8 && first < 0; c2++)
+          if (bh_live(cs[c2])) first = code * 8 + c2;
+      }
+      // next live sibling of this slot, else of the nearest ancestor that has one
+      int skip = -1, node = k, cc = c;
+      for (;;) {
+        const BhSlot *ps = slots + (size_t)node * 8;
+        for (int c2 = cc + 1; c2 < 8 && skip < 0; c2++)
+          if (bh_live(ps[c2])) skip = node * 8 + c2;
+        if (skip >= 0 || node == 0) break;
+        const int up = pslot[node];
+        node = up >> 3;
+        cc = up & 7;
+      }
+      sl[c].first = first;
+      sl[c].skip = skip;
+    }
+  }
+}
+
 // ---- force -------------------------------------------------------------------------------------
 struct BhForceArgs {
   const BhCounters *cnt;
@@ -420,152 +488,91 @@ struct BhForceArgs {
   double G, theta, soft2;
 };
 
-// The stack of one wave: entries carry the node's data, so a pop reads LDS only; opening a node
-// reads its eight child slots (384 contiguous bytes, one per lane) and pushes the live ones.
-struct BhStack {
-  double x[kBhStack], y[kBhStack], z[kBhStack], m[kBhStack], half[kBhStack];
-  unsigned long long mask[kBhStack];
-  int32_t code[kBhStack];
-};
+// One interaction, op for op as accumForce / directForce (octree.go:292-298, 314-325).
+__device__ __forceinline__ void bh_term(double dx, double dy, double dz, double distSq,
+                                        double dist, double Gm, double pm, double &fx, double &fy,
+                                        double &fz) {
+  const double force = Gm * pm / distSq;
+  fx += force * dx / dist;
+  fy += force * dy / dist;
+  fz += force * dz / dist;
+}
 
 __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
-  __shared__ BhStack stk[4];
   const int M = a.cnt->M;
   if (M < 2) return;  // a lone body feels nothing; forces were zeroed
   const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  BhStack &S = stk[w];
-  const int s = (blockIdx.x * 4 + w) * a.gsz + lane;
+  const int s = (blockIdx.x * 4 + (int)(threadIdx.x >> 6)) * a.gsz + lane;
+  bool act = lane < a.gsz && s < M;
   int i = -1;
-  bool act = false;
-  double xi = 0, yi = 0, zi = 0, Gm = 0;
-  if (lane < a.gsz && s < M) {
+  if (act) {
     i = a.perm[s];
     act = !(a.flags[i] & NB_F_SUN);  // suns sum directly
-    xi = a.sx[s];
-    yi = a.sy[s];
-    zi = a.sz[s];
-    Gm = a.G * a.sm[s];
   }
-  const unsigned long long start = __builtin_amdgcn_ballot_w64(act);
-  if (start == 0) return;
-  const double soft2 = a.soft2, theta = a.theta;
+  const unsigned long long nact = __builtin_amdgcn_ballot_w64(act);
   double fx = 0, fy = 0, fz = 0;
   unsigned long long nint = 0;
-  int sp = 0;
-  const double4 root = a.nmc[0];
-  if (root.w != 0.0) {  // accumForce: n.totalMass == 0 returns
-    if (lane == 0) {
-      S.x[0] = root.x;
-      S.y[0] = root.y;
-      S.z[0] = root.z;
-      S.m[0] = root.w;
-      S.half[0] = a.nbox[0].w;
-      S.code[0] = 0;
-      S.mask[0] = start;
-    }
-    sp = 1;
-  }
-  bh_wave_sync();
-  bool overflow = false;
-  while (sp > 0) {
-    sp--;
-    const int code = __builtin_amdgcn_readfirstlane(S.code[sp]);
-    const unsigned long long msk = S.mask[sp];
-    const bool on = (msk >> lane) & 1ull;
-    const double px = S.x[sp], py = S.y[sp], pz = S.z[sp], pm = S.m[sp], half = S.half[sp];
-    if (code <= -2) {  // a leaf: the body's centre of mass (x*m)/m and mass 0 + m
-      if (on && -2 - code != s) {
-        const double dx = px - xi;
-        const double dy = py - yi;
-        const double dz = pz - zi;
-        const double distSq = dx * dx + dy * dy + dz * dz + soft2;
-        const double dist = __builtin_sqrt(distSq);
-        const double force = Gm * pm / distSq;
-        fx += force * dx / dist;
-        fy += force * dy / dist;
-        fz += force * dz / dist;
-        nint++;
-      }
-      continue;
-    }
-    if (half < 0) {  // depth limit: the bucket's bodies one by one (directForce)
-      const int32_t *ch = a.nchild + (size_t)code * 8;
-      const int b0 = ch[1], bn = ch[2];
-      if (on)
-        for (int s2 = b0; s2 < b0 + bn; s2++) {
-          if (s2 == s) continue;
-          const double dx = a.sx[s2] - xi;
-          const double dy = a.sy[s2] - yi;
-          const double dz = a.sz[s2] - zi;
-          const double distSq = dx * dx + dy * dy + dz * dz + soft2;
-          const double dist = __builtin_sqrt(distSq);
-          const double force = Gm * a.sm[s2] / distSq;
-          fx += force * dx / dist;
-          fy += force * dy / dist;
-          fz += force * dz / dist;
-          nint++;
-        }
-      continue;
-    }
-    bool open = false;
-    if (on) {
-      const double dx = px - xi;
-      const double dy = py - yi;
-      const double dz = pz - zi;
+  if (act) {
+    const double xi = a.sx[s], yi = a.sy[s], zi = a.sz[s], Gm = a.G * a.sm[s];
+    const double soft2 = a.soft2, theta = a.theta;
+    const double4 root = a.nmc[0];
+    int q = -1;
+    if (root.w != 0.0) {  // accumForce: n.totalMass == 0 returns
+      const double dx = root.x - xi, dy = root.y - yi, dz = root.z - zi;
       const double distSq = dx * dx + dy * dy + dz * dz + soft2;
       const double dist = __builtin_sqrt(distSq);
-      if ((2 * half / dist) < theta) {
-        const double force = Gm * pm / distSq;
-        fx += force * dx / dist;
-        fy += force * dy / dist;
-        fz += force * dz / dist;
+      if ((2 * a.nbox[0].w / dist) < theta) {
+        bh_term(dx, dy, dz, distSq, dist, Gm, root.w, fx, fy, fz);
         nint++;
       } else {
-        open = true;
+        const int f = a.cnt->root_first;
+        q = f < 0 ? -1 : f;
       }
     }
-    const unsigned long long om = __builtin_amdgcn_ballot_w64(open);
-    if (om) {
-      // lane c < 8 holds child c; children with totalMass > 0 are pushed so that child 0 comes
-      // off the stack first (accumForce's loop order)
-      BhSlot cs;
-      bool valid = false;
-      if (lane < 8) {
-        cs = a.slots[(size_t)code * 8 + lane];
-        valid = cs.code != -1 && cs.m > 0;
-      }
-      const unsigned long long vm = __builtin_amdgcn_ballot_w64(valid);
-      const int nv = __builtin_popcountll(vm);
-      if (sp + nv > kBhStack) {
-        overflow = true;
-      } else {
-        if (valid) {
-          const int rank = __builtin_popcountll(vm & ((1ull << lane) - 1ull));
-          const int pos = sp + nv - 1 - rank;
-          S.x[pos] = cs.x;
-          S.y[pos] = cs.y;
-          S.z[pos] = cs.z;
-          S.m[pos] = cs.m;
-          S.half[pos] = cs.half;
-          S.code[pos] = cs.code;
-          S.mask[pos] = om;
+    // One code path for leaves and nodes, so that lanes at different kinds of slot share the
+    // square root and the term: a leaf (the body's centre of mass (x*m)/m and mass 0 + m) is
+    // taken unless it is the body itself, a node when its opening test passes.
+    // The slot's 64 bytes are read as four 16-byte words at the top of the iteration, so the
+    // walk waits for one memory round trip per visited slot.
+    while (q >= 0) {
+      const int4 *w = reinterpret_cast<const int4 *>(a.slots + q);
+      const int4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+      BhSlot sl;
+      __builtin_memcpy(&sl, &w0, 16);
+      __builtin_memcpy(reinterpret_cast<char *>(&sl) + 16, &w1, 16);
+      __builtin_memcpy(reinterpret_cast<char *>(&sl) + 32, &w2, 16);
+      __builtin_memcpy(reinterpret_cast<char *>(&sl) + 48, &w3, 16);
+      const bool leaf = sl.code <= -2;
+      const double dx = sl.x - xi, dy = sl.y - yi, dz = sl.z - zi;
+      if (!leaf && sl.half < 0) {  // depth limit: the bucket's bodies one by one (directForce)
+        const int32_t *ch = a.nchild + (size_t)sl.code * 8;
+        const int b0 = ch[1], bn = ch[2];
+        for (int s2 = b0; s2 < b0 + bn; s2++) {
+          if (s2 == s) continue;
+          const double ex = a.sx[s2] - xi, ey = a.sy[s2] - yi, ez = a.sz[s2] - zi;
+          const double distSq = ex * ex + ey * ey + ez * ez + soft2;
+          bh_term(ex, ey, ez, distSq, __builtin_sqrt(distSq), Gm, a.sm[s2], fx, fy, fz);
+          nint++;
         }
-        sp += nv;
+        q = sl.skip;
+        continue;
       }
-      bh_wave_sync();
+      const double distSq = dx * dx + dy * dy + dz * dz + soft2;
+      const double dist = __builtin_sqrt(distSq);
+      const bool take = leaf ? (-2 - sl.code != s) : ((2 * sl.half / dist) < theta);
+      if (take) {
+        bh_term(dx, dy, dz, distSq, dist, Gm, sl.m, fx, fy, fz);
+        nint++;
+      }
+      q = (leaf || take) ? sl.skip : sl.first;
     }
-  }
-  if (act) {
     a.acc[i] = fx;
     a.acc[a.npad + i] = fy;
     a.acc[2 * a.npad + i] = fz;
   }
+  if (nact == 0) return;
   for (int o = 32; o > 0; o >>= 1) nint += __shfl_xor(nint, o);
-  if (lane == 0) {
-    atomicAdd(&a.cnt_w->interactions, nint);
-    if (overflow) atomicOr(&a.cnt_w->stack_overflow, 1u);
-  }
+  if (lane == 0) atomicAdd(&a.cnt_w->interactions, nint);
 }
 
 // Suns: directForce from every other existing, non-fragmenting body in array order.
@@ -751,7 +758,8 @@ void bh_free(Bh *b) {
   if (!b) return;
   void *ps[] = {b->keys_in, b->keys, b->perm_in, b->perm, b->sx,     b->sy,     b->sz,
                 b->sm,      b->lcx,  b->lcy,     b->lcz,  b->depth,  b->suns,   b->nbox,
-                b->nmc,     b->nchild, b->nrange, b->bpart, b->sort_tmp, b->cnt, b->slots};
+                b->nmc,     b->nchild, b->nrange, b->bpart, b->sort_tmp, b->cnt, b->slots,
+                b->pslot};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   delete b;
@@ -785,6 +793,7 @@ static int bh_ensure(Ctx *c) {
   NB_HIP(c, balloc(&b->nmc, (size_t)b->node_cap));
   NB_HIP(c, balloc(&b->nchild, (size_t)b->node_cap * 8));
   NB_HIP(c, balloc(&b->slots, (size_t)b->node_cap * 8));
+  NB_HIP(c, balloc(&b->pslot, (size_t)b->node_cap));
   NB_HIP(c, balloc(&b->nrange, (size_t)b->node_cap));
   size_t bytes = 0;
   NB_HIP(c, rocprim::radix_sort_pairs(nullptr, bytes, b->keys_in, b->keys, b->perm_in, b->perm,
@@ -853,6 +862,11 @@ int bh_force_phase(Ctx *c, bool timed, int ev_slot) {
     ma.d = d;
     hipLaunchKernelGGL(k_bh_mass, dim3(gl), dim3(256), 0, st, ma);
   }
+  const int gn = std::min(4096, nb256(std::max(1, n)));  // nodes < n
+  hipLaunchKernelGGL(k_bh_parent, dim3(gn), dim3(256), 0, st, b->cnt, b->nchild, D, b->node_cap,
+                     b->pslot);
+  hipLaunchKernelGGL(k_bh_links, dim3(gn), dim3(256), 0, st, b->cnt, b->nchild, b->pslot, D,
+                     b->node_cap, b->slots);
   NB_HIP(c, hipGetLastError());
   BhForceArgs fa;
   fa.cnt = b->cnt;
