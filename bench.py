@@ -45,7 +45,7 @@ WORKLOADS = {
     "collide2x524k": dict(n=1048576, desc="two Plummer spheres of 524,288 bodies (masses 1:3, "
                                           "offset 10a, approaching at 0.5), elastic collisions, "
                                           "radii 0.05 m^(1/3), eps=0.01, dt=1e-3, 200 steps, "
-                                          "40^3 grid, shell 3: "
+                                          "64^3 grid, shell 3: "
                                           "surrogate far field + exact near field (detection "
                                           "there) against the plain FP32 direct sum; energy and "
                                           "momentum drift from FP64 diagnostics"),
@@ -118,7 +118,7 @@ def parse():
     ap.add_argument("--c5-tol", type=float, default=1e-3,
                     help="sampled-error tolerance of the surrogate in the collision run")
     ap.add_argument("--c5-train-sims", type=int, default=8, help="training sims, collision run")
-    ap.add_argument("--c5-grid", type=int, default=40, help="surrogate cells per axis, collision run")
+    ap.add_argument("--c5-grid", type=int, default=64, help="surrogate cells per axis, collision run")
     ap.add_argument("--c5-shell", type=int, default=3, help="surrogate shell depth, collision run")
     return ap.parse_args()
 

@@ -355,7 +355,7 @@ int nb_mlp_train(nb_mlp *m, const float *x, const float *t, int64_t rows, int32_
  * needs every radius <= r0 / grid, so that each overlapping pair lies in adjacent cells.
  * nb_stats.interactions still counts the N(N-1) pairs the step stands for. */
 typedef struct nb_sur_params {
-  int32_t grid;          /* cells per axis, 4..40 (32) */
+  int32_t grid;          /* cells per axis, 4..64 (32) */
   int32_t shell;         /* per-body multipole shell depth K, 1..6 (2) */
   double r0;             /* stretch radius; 0 = 1.5 x the half-mass radius at enable time */
   int32_t use_mlp;       /* apply the learned residual once trained (1) */

@@ -41,7 +41,9 @@ int mlp_train_step_device(nb_mlp *m, const float *x_dev, const float *t_dev, int
 constexpr int kCellRec = 16;  // COM(3), M, Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, radius, ix, iy, iz, n, 0
 constexpr int kFarRec = 20;   // a(3), J(6: xx yy zz xy xz yz), H(10), 0
 constexpr int kFeat = 30;     // MLP input width
-constexpr int kMaxGrid = 40;  // 40^3 + 1 keys fit the 16-bit radix sort
+constexpr int kMaxGrid = 64;  // cell keys 0..64^3 (the last for inactive bodies): 19 bits
+// Bits of the cell keys 0..ncell sorted by the radix sort (16 up to 40^3 cells).
+static unsigned key_bits(int ncell) { return 32u - (unsigned)__builtin_clz((unsigned)ncell); }
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(4))) f4v *cf4p;  // scalar-cache (uniform) loads
@@ -1037,7 +1039,8 @@ static int sur_ensure_bodies(Ctx *c) {
   NB_HIP(c, salloc(&s->feat, n * kFeat));
   size_t bytes = 0;
   NB_HIP(c, rocprim::radix_sort_pairs(nullptr, bytes, s->keys, s->keys2, s->perm_in, s->perm,
-                                      (size_t)n, 0u, 16u, c->stream));
+                                      (size_t)n, 0u, key_bits(kMaxGrid * kMaxGrid * kMaxGrid),
+                                      c->stream));
   void *tmp = nullptr;
   NB_HIP(c, hipMalloc(&tmp, bytes ? bytes : 1));
   if (s->sort_tmp) (void)hipFree(s->sort_tmp);
@@ -1101,7 +1104,7 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
   NB_HIP(c, hipGetLastError());
   size_t bytes = s->sort_bytes;
   NB_HIP(c, rocprim::radix_sort_pairs(s->sort_tmp, bytes, s->keys, s->keys2, s->perm_in, s->perm,
-                                      (size_t)n, 0u, 16u, st));
+                                      (size_t)n, 0u, key_bits(ncell), st));
   hipLaunchKernelGGL(k_sur_bounds, dim3((unsigned)(ncell / 1024 + 1)), dim3(1024), 0, st,
                      s->keys2, n, ncell, s->cstart, s->btot);
   hipLaunchKernelGGL(k_sur_scan, dim3((unsigned)((ncell + 1023) / 1024)), dim3(1024), 0, st,

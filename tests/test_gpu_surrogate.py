@@ -128,7 +128,7 @@ def test_unsupported_configurations_are_rejected():
             assert e.value.code == code
     with sim() as g:
         g.load(b)
-        for bad in (dict(grid=2), dict(grid=41), dict(shell=0), dict(tol=0.0), dict(r0=-1.0)):
+        for bad in (dict(grid=2), dict(grid=65), dict(shell=0), dict(tol=0.0), dict(r0=-1.0)):
             with pytest.raises(NBodyError) as e:
                 g.sur_enable(**bad)
             assert e.value.code == L.NB_E_INVALID
@@ -173,14 +173,15 @@ def test_learned_residual_gain_on_held_out_sims(tmp_path):
 # ---------------------------------------------------------------------------------------------
 # two spheres, and collisions in the near field
 # ---------------------------------------------------------------------------------------------
-def test_two_spheres_error(oracle):
+@pytest.mark.parametrize("grid", [32, 64])  # 64^3 cells: 19-bit cell keys
+def test_two_spheres_error(oracle, grid):
     b = gen.two_plummer(65536, seed=21)
     st = oracle.State.from_bodies(b)
     idx = np.sort(np.random.default_rng(1).choice(b.n, 512, replace=False))
     ref = oracle.force_soft(st, 1.0, EPS, idx=idx, nthreads=NT)
     with sim() as g:
         g.load(b)
-        g.sur_enable(check_every=0, use_mlp=False)
+        g.sur_enable(grid=grid, check_every=0, use_mlp=False)
         err = glob_err(g.forces()[idx], ref)
         st2 = g.sur_stats()
     assert err <= 3e-3, err
@@ -197,8 +198,9 @@ def test_collisions_need_radii_within_a_cell():
         g.sur_enable()
 
 
-@pytest.mark.parametrize("radius_k,mixed", [(0.2, False), (1.0, True)])
-def test_near_field_detection_matches_oracle_pairs(oracle, radius_k, mixed):
+@pytest.mark.parametrize("radius_k,mixed,grid", [(0.2, False, 32), (1.0, True, 32),
+                                                 (0.2, False, 64)])
+def test_near_field_detection_matches_oracle_pairs(oracle, radius_k, mixed, grid):
     """Every overlapping pair, exactly the oracle's set (FP64, d <= Ri + Rj), with dense overlaps
     (about 30,000 pairs at radius_k = 1), removed bodies and every collision behaviour."""
     b = gen.two_plummer(20000, seed=22, radius_k=radius_k)
@@ -209,7 +211,7 @@ def test_near_field_detection_matches_oracle_pairs(oracle, radius_k, mixed):
     st = oracle.State.from_bodies(b)
     with NBodyGPU(G=1.0, eps=EPS, overlap_cutoff=False, collisions=True) as g:
         g.load(b)
-        g.sur_enable(check_every=0)
+        g.sur_enable(grid=grid, check_every=0)
         g.forces()
         pairs = g.pairs()
     exp = oracle.detect(st)
