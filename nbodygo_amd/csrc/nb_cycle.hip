@@ -38,33 +38,53 @@ __device__ static void emit(uint64_t *pairs, DevCounters *ctr, int64_t cap, int 
 //       distSq = d2 + eps*eps; dist = sqrt(distSq); force = G*mi*mj/distSq; fx += force*dx/dist
 // Pairs are emitted from the lower index (Collided, body.go:192-208).
 // ---------------------------------------------------------------------------------------------
+// Sources are staged in LDS 256 at a time by the workgroup and read back as broadcasts; each
+// lane still sums its sources in array order, op for op as calcForceFrom.
 __global__ __launch_bounds__(256) void k_force_f64(Force64Args a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.npad) return;
+  __shared__ double sx[256], sy[256], sz[256], sm[256], sr[256];
+  __shared__ int32_t sb[256];
+  __shared__ uint8_t sf[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // npad is a multiple of 256
   double fx = 0, fy = 0, fz = 0;
   const bool ai = i < a.n && active_f(a.flags[i]);
-  if (__builtin_amdgcn_ballot_w64(ai) != 0) {
-    const double xi = ai ? a.x[i] : 0, yi = ai ? a.y[i] : 0, zi = ai ? a.z[i] : 0;
-    const double mi = ai ? a.m[i] : 0, ri = ai ? a.r[i] : 0;
-    const int32_t bi = ai ? a.beh[i] : NB_CB_NONE;
-    const bool ci = ai && nb_may_pair(bi, a.events);
-    const double G = a.G, eps2 = a.eps * a.eps;
-    const bool ref = a.eps == 0.0 && a.cutoff;
-    cdp X = (cdp)a.x, Y = (cdp)a.y, Z = (cdp)a.z, M = (cdp)a.m, Rr = (cdp)a.r;
-    cip B = (cip)a.beh;
-    for (int j = 0; j < a.n; j++) {
-      if (!active_f(a.flags[j])) continue;
-      const double dx = X[j] - xi;
-      const double dy = Y[j] - yi;
-      const double dz = Z[j] - zi;
+  const bool wave_on = __builtin_amdgcn_ballot_w64(ai) != 0;
+  const double xi = ai ? a.x[i] : 0, yi = ai ? a.y[i] : 0, zi = ai ? a.z[i] : 0;
+  const double mi = ai ? a.m[i] : 0, ri = ai ? a.r[i] : 0;
+  const int32_t bi = ai ? a.beh[i] : NB_CB_NONE;
+  const bool ci = ai && nb_may_pair(bi, a.events);
+  const double G = a.G, eps2 = a.eps * a.eps;
+  const bool ref = a.eps == 0.0 && a.cutoff;
+  // the unsoftened distance is only needed for the overlap gate and for detection
+  const bool need_d0 = ref || a.cutoff || a.collisions;
+  for (int t0 = 0; t0 < a.n; t0 += 256) {
+    __syncthreads();
+    const int js = t0 + threadIdx.x;
+    if (js < a.n) {
+      sx[threadIdx.x] = a.x[js];
+      sy[threadIdx.x] = a.y[js];
+      sz[threadIdx.x] = a.z[js];
+      sm[threadIdx.x] = a.m[js];
+      sr[threadIdx.x] = a.r[js];
+      sb[threadIdx.x] = a.beh[js];
+      sf[threadIdx.x] = a.flags[js];
+    }
+    __syncthreads();
+    if (!wave_on) continue;
+    const int cnt = min(256, a.n - t0);
+    for (int jj = 0; jj < cnt; jj++) {
+      if (!active_f(sf[jj])) continue;
+      const int j = t0 + jj;
+      const double dx = sx[jj] - xi;
+      const double dy = sy[jj] - yi;
+      const double dz = sz[jj] - zi;
       const double d2 = dx * dx + dy * dy + dz * dz;
-      const double rs = ri + Rr[j];
+      const double rs = ri + sr[jj];
       if (j != i && ai) {
-        const double dist0 = __builtin_sqrt(d2);
-        const bool overlap = !(dist0 > rs);
+        const double dist0 = need_d0 ? __builtin_sqrt(d2) : 0.0;
+        const bool overlap = need_d0 && !(dist0 > rs);
         if (ref) {
           if (!overlap) {
-            const double force = G * mi * M[j] / (dist0 * dist0);
+            const double force = G * mi * sm[jj] / (dist0 * dist0);
             fx += force * dx / dist0;
             fy += force * dy / dist0;
             fz += force * dz / dist0;
@@ -73,14 +93,14 @@ __global__ __launch_bounds__(256) void k_force_f64(Force64Args a) {
           if (!(a.cutoff && overlap)) {
             const double distSq = d2 + eps2;
             const double dist = __builtin_sqrt(distSq);
-            const double force = G * mi * M[j] / distSq;
+            const double force = G * mi * sm[jj] / distSq;
             fx += force * dx / dist;
             fy += force * dy / dist;
             fz += force * dz / dist;
           }
         }
         if (a.collisions && overlap && i < j && ci &&
-            nb_pair_event(bi, B[j], ri, Rr[j], dist0, a.events))
+            nb_pair_event(bi, sb[jj], ri, sr[jj], dist0, a.events))
           emit(a.pairs, a.ctr, a.pair_cap, i, j);
       }
     }
