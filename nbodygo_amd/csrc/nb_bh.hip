@@ -42,7 +42,7 @@
 namespace nb {
 
 constexpr int kBhLevels = 21;        // octal digits in a 64-bit key
-constexpr int kBhStack = 160;        // per-wave stack entries (bound: 7 * 20 + 8 = 148)
+constexpr int kBhStack = 160;        // collision walk: per-wave stack entries (bound 7 * 20 + 8)
 constexpr int32_t kBucket = INT32_MIN;
 constexpr int kSunChunk = 64;
 
