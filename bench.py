@@ -405,8 +405,8 @@ def cpu_baseline(b, seconds):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (per-GPU sharding is described in "
-              "DESIGN.md); got --gpus %d" % a.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU only (sharding the targets over several GPUs is not "
+              "built); got --gpus %d" % a.gpus, file=sys.stderr)
         sys.exit(2)
     from nbodygo_amd import NBodyGPU, gen
 
