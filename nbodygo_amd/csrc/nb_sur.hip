@@ -640,7 +640,7 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   float4 *tl = tiles[wave];
   float4 (*rl)[3] = shrec[wave];
   double ax = 0, ay = 0, az = 0;
-  int chunk = 0, nsrc = 0;
+  int nsrc = 0;
   uint32_t i_body = 0;
   float ri = 0.f, me_inf = 0.f;
   if (COLL && tv) {
@@ -648,15 +648,36 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
     ri = round_up_f(a.r[i_body]);
     me_inf = fmaxf(fabsf(me.x), fmaxf(fabsf(me.y), fabsf(me.z)));
   }
+  // The sources are the bodies of the 3 x 3 rows of three adjacent cells around the cell, each
+  // row contiguous in the sorted order.  Taken in row order they are dealt to the four waves as
+  // four equal contiguous shares (so a wave's work does not depend on how the rows split into
+  // chunks), and each share is walked in chunks of up to 64 that stay within one row.
   const int xl = max(cx - 1, 0), xh = min(cx + 1, G - 1);
-  for (int z = max(cz - 1, 0); z <= min(cz + 1, G - 1); z++)
-    for (int y = max(cy - 1, 0); y <= min(cy + 1, G - 1); y++) {
+  auto row_range = [&](int q, int &b, int &l) {
+    const int y = cy - 1 + q % 3, z = cz - 1 + q / 3;
+    b = l = 0;
+    if (y >= 0 && y < G && z >= 0 && z < G) {
       const int row = G * (y + G * z);
-      const int r0i = cs[xl + row], r1i = cs[xh + row + 1];
-      nsrc += r1i - r0i;
-      for (int s0 = r0i; s0 < r1i; s0 += 64, chunk++) {
-        if ((chunk & 3) != wave) continue;
-        const int cnt = min(64, r1i - s0);
+      b = cs[xl + row];
+      l = cs[xh + row + 1] - b;
+    }
+  };
+  for (int q = 0; q < 9; q++) {
+    int b, l;
+    row_range(q, b, l);
+    nsrc += l;
+  }
+  const int share = (nsrc + 3) / 4;
+  const int w0 = wave * share, w1 = min(nsrc, w0 + share);
+  int base = 0;
+  for (int q = 0; q < 9; q++) {
+    int rb, rlen;
+    row_range(q, rb, rlen);
+    const int lo = max(w0, base), hi = min(w1, base + rlen);
+    for (int s = lo; s < hi; s += 64) {
+      {
+        const int s0 = rb + (s - base);
+        const int cnt = min(64, hi - s);
         // the wave's private tile: one source per lane, zero-mass padding to a multiple of 4
         const float4 src = lane < cnt ? a.sp[s0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
         tl[lane] = src;
@@ -716,6 +737,8 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
         az += pz;
       }
     }
+    base += rlen;
+  }
   // multipole shell, Chebyshev distance 2..K: the nonempty cells of the (2K+1)^3 box, listed by
   // ballot (one box position per lane, 64 per round) in box order and dealt round robin to the
   // waves; each wave stages its records in LDS (one per lane) and reads them back as broadcasts.
