@@ -325,7 +325,10 @@ int nb_create(const nb_params *p, nb_ctx **out) {
   if (!hip_ok(nullptr, hipSetDevice(c->device), "hipSetDevice") ||
       !hip_ok(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking),
               "hipStreamCreate") ||
-      !hip_ok(nullptr, hipMalloc((void **)&c->ctr, sizeof(DevCounters)), "hipMalloc")) {
+      !hip_ok(nullptr, hipMalloc((void **)&c->ctr, sizeof(DevCounters)), "hipMalloc") ||
+      !hip_ok(nullptr, hipMalloc((void **)&c->small, 8 * sizeof(unsigned long long)),
+              "hipMalloc")) {
+    if (c->ctr) (void)hipFree(c->ctr);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return NB_E_HIP;
@@ -346,6 +349,7 @@ void nb_destroy(nb_ctx *c) {
   if (c->ev_scan_tmp) (void)hipFree(c->ev_scan_tmp);
   c->ev_scan_tmp = nullptr;
   free_all(c);
+  if (c->small) (void)hipFree(c->small);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -553,18 +557,14 @@ int nb_compute_forces(nb_ctx *c, double *fx, double *fy, double *fz) {
   NB_HIP(c, hipSetDevice(c->device));
   int rc = force_phase(c, false, 0);
   if (rc) return rc;
-  double *tmp = nullptr;
-  NB_HIP(c, hipMalloc((void **)&tmp, (size_t)3 * c->n * sizeof(double)));
-  hipError_t e = launch_combine_force(c->acc, split_now(c), (int)c->npad, (int)c->n, c->m,
-                                      c->p.G, c->p.precision == NB_PREC_FP64 ? 1 : 0, tmp,
-                                      c->stream);
+  double *tmp = c->diag;  // 3 n doubles of the context's scratch
+  NB_HIP(c, launch_combine_force(c->acc, split_now(c), (int)c->npad, (int)c->n, c->m, c->p.G,
+                                 c->p.precision == NB_PREC_FP64 ? 1 : 0, tmp, c->stream));
   const size_t db = (size_t)c->n * sizeof(double);
-  if (e == hipSuccess) e = hipMemcpyAsync(fx, tmp, db, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(fy, tmp + c->n, db, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(fz, tmp + 2 * c->n, db, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(tmp);
-  NB_HIP(c, e);
+  NB_HIP(c, hipMemcpyAsync(fx, tmp, db, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipMemcpyAsync(fy, tmp + c->n, db, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipMemcpyAsync(fz, tmp + 2 * c->n, db, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
   DevCounters ctr;
   NB_HIP(c, hipMemcpy(&ctr, c->ctr, sizeof ctr, hipMemcpyDeviceToHost));
   if (ctr.pair_overflow)
@@ -582,33 +582,32 @@ int nb_accel_fp64(nb_ctx *c, const int64_t *idx, int64_t nidx, double *ax, doubl
   for (int64_t k = 0; k < nidx; k++)
     if (idx[k] < 0 || idx[k] >= c->n) return fail(c, NB_E_INVALID, "nb_accel_fp64: index out of range");
   NB_HIP(c, hipSetDevice(c->device));
-  int64_t *didx = nullptr;
-  double *dout = nullptr;
-  NB_HIP(c, hipMalloc((void **)&didx, (size_t)nidx * sizeof(int64_t)));
-  hipError_t e = hipMalloc((void **)&dout, (size_t)3 * nidx * sizeof(double));
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(didx, idx, (size_t)nidx * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) {
+  // staged through the context's scratch, in chunks of diag_len / 4 targets (an index and three
+  // outputs each)
+  const int64_t chunk = c->diag_len / 4;
+  int64_t *didx = reinterpret_cast<int64_t *>(c->diag);
+  double *dout = c->diag + chunk;
+  for (int64_t k0 = 0; k0 < nidx; k0 += chunk) {
+    const int64_t k = std::min(chunk, nidx - k0);
+    NB_HIP(c, hipMemcpyAsync(didx, idx + k0, (size_t)k * sizeof(int64_t), hipMemcpyHostToDevice,
+                             c->stream));
     Accel64Args a;
     a.x = c->x; a.y = c->y; a.z = c->z; a.m = c->m; a.r = c->r;
     a.flags = c->flags;
     a.idx = didx;
-    a.nidx = nidx;
+    a.nidx = k;
     a.out = dout;
     a.n = (int)c->n;
     a.G = c->p.G;
     a.eps = c->p.eps;
     a.cutoff = c->p.overlap_cutoff;
-    e = launch_accel_f64(a, c->stream);
+    NB_HIP(c, launch_accel_f64(a, c->stream));
+    const size_t db = (size_t)k * sizeof(double);
+    NB_HIP(c, hipMemcpyAsync(ax + k0, dout, db, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipMemcpyAsync(ay + k0, dout + k, db, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipMemcpyAsync(az + k0, dout + 2 * k, db, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
   }
-  const size_t db = (size_t)nidx * sizeof(double);
-  if (e == hipSuccess) e = hipMemcpyAsync(ax, dout, db, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(ay, dout + nidx, db, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(az, dout + 2 * nidx, db, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(didx);
-  if (dout) (void)hipFree(dout);
-  NB_HIP(c, e);
   return NB_OK;
 }
 
@@ -681,14 +680,11 @@ int nb_store_render(nb_ctx *c, float *xyz, uint8_t *exists) {
   if (c->n == 0) return NB_OK;
   if (!xyz) return fail(c, NB_E_INVALID, "nb_store_render: NULL xyz");
   NB_HIP(c, hipSetDevice(c->device));
-  float *tmp = nullptr;
-  NB_HIP(c, hipMalloc((void **)&tmp, (size_t)3 * c->n * sizeof(float)));
-  hipError_t e = launch_render(c->x, c->y, c->z, c->flags, (int)c->n, tmp, c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(xyz, tmp, (size_t)3 * c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(tmp);
-  NB_HIP(c, e);
+  float *tmp = reinterpret_cast<float *>(c->diag);  // 3 n floats of the context's scratch
+  NB_HIP(c, launch_render(c->x, c->y, c->z, c->flags, (int)c->n, tmp, c->stream));
+  NB_HIP(c, hipMemcpyAsync(xyz, tmp, (size_t)3 * c->n * sizeof(float), hipMemcpyDeviceToHost,
+                           c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
   if (exists) {
     std::vector<uint8_t> f((size_t)c->n);
     NB_HIP(c, hipMemcpy(f.data(), c->flags, (size_t)c->n, hipMemcpyDeviceToHost));
@@ -807,14 +803,11 @@ int nb_autotune(nb_ctx *c, int32_t reps, int32_t *cfg) {
 // The largest radius over active bodies (the value nb_load computes on the host), from the device
 // arrays after a removal, compaction or patch.
 static int refresh_rmax(Ctx *c) {
-  unsigned long long *d = nullptr, h = 0;
-  NB_HIP(c, hipMalloc((void **)&d, sizeof h));
-  hipError_t e = hipMemsetAsync(d, 0, sizeof h, c->stream);
-  if (e == hipSuccess) e = launch_rmax_active(c->r, c->flags, (int)c->n, d, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(d);
-  NB_HIP(c, e);
+  unsigned long long *d = c->small, h = 0;
+  NB_HIP(c, hipMemsetAsync(d, 0, sizeof h, c->stream));
+  NB_HIP(c, launch_rmax_active(c->r, c->flags, (int)c->n, d, c->stream));
+  NB_HIP(c, hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
   double v;
   memcpy(&v, &h, sizeof v);
   c->rmax = v;
@@ -1033,19 +1026,14 @@ int nb_remove_bodies(nb_ctx *c, int64_t count, int64_t *removed) {
   const bool all = count == -1;
   const int64_t step = all ? 1 : (count > n ? 1 : n / count);
   const int64_t npoints = all ? n : (n + step - 1) / step;
-  unsigned long long *cnt = nullptr, h[2] = {0, 0};
-  NB_HIP(c, hipMalloc((void **)&cnt, sizeof h));
-  hipError_t e = hipMemsetAsync(cnt, 0, sizeof h, c->stream);
-  if (e == hipSuccess)
-    e = launch_remove_walk(c->flags, c->m, (int)n, (int)step, (int)npoints, all ? 1 : 0, cnt,
-                           c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess)
-    e = launch_write_stream(c->x, c->y, c->z, c->m, c->flags, c->js, (int)n, (int)c->npad,
-                            c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(cnt);
-  NB_HIP(c, e);
+  unsigned long long *cnt = c->small, h[2] = {0, 0};
+  NB_HIP(c, hipMemsetAsync(cnt, 0, sizeof h, c->stream));
+  NB_HIP(c, launch_remove_walk(c->flags, c->m, (int)n, (int)step, (int)npoints, all ? 1 : 0, cnt,
+                               c->stream));
+  NB_HIP(c, hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, launch_write_stream(c->x, c->y, c->z, c->m, c->flags, c->js, (int)n, (int)c->npad,
+                                c->stream));
+  NB_HIP(c, hipStreamSynchronize(c->stream));
   c->n_active = std::max<int64_t>(0, c->n_active - (int64_t)h[1]);
   if (removed) *removed = (int64_t)h[0];
   return refresh_rmax(c);

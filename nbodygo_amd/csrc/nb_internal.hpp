@@ -123,8 +123,11 @@ struct Ctx {
   int64_t pair_cap_pow2 = 0;
 
   DevCounters *ctr = nullptr;
-  double *diag = nullptr;  // scratch for reductions
+  // scratch of cap * 8 + 64 doubles: the diagnostics' reduction, and the staging area of
+  // nb_compute_forces, nb_store_render and nb_accel_fp64 (each call completes before it returns)
+  double *diag = nullptr;
   int64_t diag_len = 0;
+  unsigned long long *small = nullptr;  // 8 counters for single-value reductions (context-long)
 
   ForceShape shape;
   bool shape_tuned = false;
