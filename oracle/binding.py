@@ -79,6 +79,9 @@ def lib():
         L.or_bench_workpool.argtypes = [sp, C.c_double, C.c_int64, C.c_int, C.c_int, _dp,
                                         C.POINTER(C.c_int64)]
         L.or_bench_workpool.restype = C.c_double
+        L.or_bench_cycles.argtypes = [sp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.POINTER(OrStats)]
+        L.or_bench_cycles.restype = C.c_double
         L.or_bh_force.argtypes = [sp, C.POINTER(OrBhParams), _dp, _dp, _dp,
                                   C.POINTER(C.c_int32), C.POINTER(OrBhInfo)]
         L.or_bh_force.restype = C.c_int
@@ -215,6 +218,16 @@ def bench_workpool(st, G, ntargets, nsteps=1, nthreads=8, want_force=False):
     el = lib().or_bench_workpool(C.byref(st.s), G, ntargets, nsteps, nthreads,
                                  f.ctypes.data_as(_dp) if want_force else None, C.byref(hits))
     return el, hits.value, f
+
+
+def bench_cycles(st, G, dt, R, nsteps, nthreads=8):
+    """nsteps whole cycles of the reference's CPU algorithm (or_bench_cycles) on st, in place:
+    (elapsed seconds, stats dict)."""
+    stats = OrStats()
+    el = lib().or_bench_cycles(C.byref(st.s), G, dt, R, nsteps, nthreads, C.byref(stats))
+    if el < 0:
+        raise RuntimeError("or_bench_cycles failed")
+    return el, {k[0]: getattr(stats, k[0]) for k in OrStats._fields_}
 
 
 def bh_force(st, G=6.673e-11, theta=0.3, soft=1e-4, max_depth=21):

@@ -643,3 +643,209 @@ double or_bench_workpool(const or_state *s, double G, int64_t ntargets, int nste
   free(store);
   return el;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Whole cycles of the reference's CPU algorithm (timing baseline for the 1,000-body scene)    */
+/* ------------------------------------------------------------------------------------------ */
+/* One worker of the cycle pool: Body.Compute (body.go:148-187) on its slice of targets, the
+ * force loop then the detection loop over IterateOnce.  Where the reference enqueues
+ * newCollision(b, o) for an overlapping Elastic/Fragment pair (body.go:172-177) the worker keeps
+ * the unordered pair once, from its lower index, for the canonical resolution. */
+typedef struct {
+  wp_body **arr;
+  int64_t n;
+  double G;
+  int64_t lo, hi;
+  int32_t *pairs;
+  int64_t np, cap;
+  int nsteps, fail;
+  pthread_barrier_t *start, *done;
+} cy_worker;
+
+static void cy_push(cy_worker *w, int64_t i, int64_t j) {
+  if (w->np == w->cap) {
+    int64_t c = w->cap ? 2 * w->cap : 64;
+    int32_t *p = (int32_t *)realloc(w->pairs, sizeof(int32_t) * 2 * (size_t)c);
+    if (!p) {
+      w->fail = 1;
+      return;
+    }
+    w->pairs = p;
+    w->cap = c;
+  }
+  w->pairs[2 * w->np] = (int32_t)i;
+  w->pairs[2 * w->np + 1] = (int32_t)j;
+  w->np++;
+}
+
+static void cy_compute(cy_worker *w, int64_t i) {
+  wp_body *b = w->arr[i];
+  wp_body **arr = w->arr;
+  const int64_t n = w->n;
+  const double G = w->G;
+  b->fx = b->fy = b->fz = 0;
+  if (!b->exists || b->fragmenting) return;
+  for (int64_t k = 0; k < n; k++) { /* calcForceFrom, body.go:214-225 */
+    wp_body *o = arr[k];
+    if (b == o || !o->exists || o->fragmenting) continue;
+    double dx = o->X - b->X;
+    double dy = o->Y - b->Y;
+    double dz = o->Z - b->Z;
+    double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (b->collided || dist > b->Radius + o->Radius) {
+      double force = G * b->Mass * o->Mass / (dist * dist);
+      b->fx += force * dx / dist;
+      b->fy += force * dy / dist;
+      b->fz += force * dz / dist;
+    }
+  }
+  if (b->collided || !can_collide(b->beh)) return;
+  for (int64_t k = i + 1; k < n; k++) { /* Collided, body.go:192-208 */
+    wp_body *o = arr[k];
+    if (!o->exists || o->fragmenting || !can_collide(o->beh)) continue;
+    double dx = o->X - b->X;
+    double dy = o->Y - b->Y;
+    double dz = o->Z - b->Z;
+    double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(dist > b->Radius + o->Radius)) cy_push(w, i, k);
+  }
+}
+
+static void *cy_thread(void *a) {
+  cy_worker *w = (cy_worker *)a;
+  for (int s = 0; s < w->nsteps; s++) {
+    pthread_barrier_wait(w->start);
+    w->np = 0;
+    for (int64_t i = w->lo; i < w->hi; i++) cy_compute(w, i);
+    pthread_barrier_wait(w->done);
+  }
+  return NULL;
+}
+
+/* ProcessMods on the AoS bodies: each canonical pair resolved by or_resolve on a two-body view,
+ * so the arithmetic is the oracle's own. */
+static void cy_resolve(wp_body **arr, const int32_t *pairs, int64_t np, or_stats *st) {
+  for (int64_t k = 0; k < np; k++) {
+    wp_body *b[2] = {arr[pairs[2 * k]], arr[pairs[2 * k + 1]]};
+    double x[2], y[2], z[2], vx[2], vy[2], vz[2], m[2], r[2], rest[2], frag[2] = {0, 0};
+    int32_t beh[2];
+    uint8_t flags[2], col[2];
+    for (int q = 0; q < 2; q++) {
+      x[q] = b[q]->X; y[q] = b[q]->Y; z[q] = b[q]->Z;
+      vx[q] = b[q]->Vx; vy[q] = b[q]->Vy; vz[q] = b[q]->Vz;
+      m[q] = b[q]->Mass; r[q] = b[q]->Radius; rest[q] = b[q]->r;
+      beh[q] = b[q]->beh;
+      flags[q] = (uint8_t)((b[q]->exists ? OR_F_EXISTS : 0) | (b[q]->fragmenting ? OR_F_FRAGMENTING : 0));
+      col[q] = b[q]->collided;
+    }
+    or_state v = {2, x, y, z, vx, vy, vz, m, r, rest, frag, beh, flags, col};
+    const int32_t p01[2] = {0, 1};
+    or_resolve(&v, p01, 1, st);
+    for (int q = 0; q < 2; q++) {
+      b[q]->Vx = vx[q]; b[q]->Vy = vy[q]; b[q]->Vz = vz[q];
+      b[q]->collided = col[q];
+    }
+  }
+}
+
+double or_bench_cycles(or_state *s, double G, double dt, double R, int nsteps, int nthreads,
+                       or_stats *st) {
+  int64_t n = s->n;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  wp_body *store = (wp_body *)calloc((size_t)(n > 0 ? n : 1), sizeof(wp_body));
+  wp_body **arr = (wp_body **)malloc(sizeof(wp_body *) * (size_t)(n > 0 ? n : 1));
+  if (!store || !arr) {
+    free(store);
+    free(arr);
+    return -1.0;
+  }
+  for (int64_t i = 0; i < n; i++) {
+    wp_body *b = &store[i];
+    b->X = s->x[i]; b->Y = s->y[i]; b->Z = s->z[i];
+    b->Vx = s->vx[i]; b->Vy = s->vy[i]; b->Vz = s->vz[i];
+    b->Radius = s->r[i];
+    b->Mass = s->m[i];
+    b->r = s->rest[i];
+    b->beh = s->beh[i];
+    b->exists = (s->flags[i] & OR_F_EXISTS) ? 1 : 0;
+    b->fragmenting = (s->flags[i] & OR_F_FRAGMENTING) ? 1 : 0;
+    b->collided = s->collided ? s->collided[i] : 0;
+    arr[i] = b;
+  }
+  /* slices as runOneComputation sizes them (computation-runner.go:285-305) */
+  int64_t size = n / nthreads;
+  if (n < 100 || size == 0) size = n > 0 ? n : 1;
+  int nw = (int)((n + size - 1) / size);
+  if (nw < 1) nw = 1;
+  pthread_barrier_t start, done;
+  pthread_barrier_init(&start, NULL, (unsigned)nw + 1);
+  pthread_barrier_init(&done, NULL, (unsigned)nw + 1);
+  cy_worker *ws = (cy_worker *)calloc((size_t)nw, sizeof(cy_worker));
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nw);
+  for (int t = 0; t < nw; t++) {
+    ws[t].arr = arr;
+    ws[t].n = n;
+    ws[t].G = G;
+    ws[t].lo = (int64_t)t * size;
+    ws[t].hi = ws[t].lo + size > n ? n : ws[t].lo + size;
+    ws[t].nsteps = nsteps;
+    ws[t].start = &start;
+    ws[t].done = &done;
+    pthread_create(&th[t], NULL, cy_thread, &ws[t]);
+  }
+  double t0 = now_s();
+  for (int step = 0; step < nsteps; step++) {
+    pthread_barrier_wait(&start); /* submitSlice for every slice */
+    pthread_barrier_wait(&done);  /* wait */
+    int64_t np = 0;
+    for (int t = 0; t < nw; t++) { /* slices in order: canonical (i, j) order */
+      cy_resolve(arr, ws[t].pairs, ws[t].np, st);
+      np += ws[t].np;
+    }
+    if (st) {
+      st->pairs_total += np;
+      st->pairs_last = np;
+    }
+    for (int64_t i = 0; i < n; i++) { /* the Update loop, body.go:114-139 */
+      wp_body *b = arr[i];
+      if (!b->exists) continue;
+      if (!b->collided) {
+        b->Vx += dt * b->fx / b->Mass;
+        b->Vy += dt * b->fy / b->Mass;
+        b->Vz += dt * b->fz / b->Mass;
+      }
+      b->X += dt * b->Vx;
+      b->Y += dt * b->Vy;
+      b->Z += dt * b->Vz;
+      b->collided = 0;
+      b->r = R;
+      if (isnan(b->X) || isnan(b->Y) || isnan(b->Z)) {
+        b->exists = 0;
+        if (st) st->removed_total++;
+      }
+    }
+  }
+  double el = now_s() - t0;
+  int fail = 0;
+  for (int t = 0; t < nw; t++) {
+    pthread_join(th[t], NULL);
+    fail |= ws[t].fail;
+    free(ws[t].pairs);
+  }
+  for (int64_t i = 0; i < n; i++) {
+    const wp_body *b = arr[i];
+    s->x[i] = b->X; s->y[i] = b->Y; s->z[i] = b->Z;
+    s->vx[i] = b->Vx; s->vy[i] = b->Vy; s->vz[i] = b->Vz;
+    s->rest[i] = b->r;
+    s->flags[i] = (uint8_t)((s->flags[i] & ~OR_F_EXISTS) | (b->exists ? OR_F_EXISTS : 0));
+    if (s->collided) s->collided[i] = b->collided;
+  }
+  pthread_barrier_destroy(&start);
+  pthread_barrier_destroy(&done);
+  free(th);
+  free(ws);
+  free(arr);
+  free(store);
+  return fail ? -1.0 : el;
+}

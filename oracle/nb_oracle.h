@@ -122,6 +122,16 @@ void or_energy(const or_state *s, double G, double eps, double *out, int nthread
 double or_bench_workpool(const or_state *s, double G, int64_t ntargets, int nsteps,
                          int nthreads, double *fx_out, int64_t *hits_out);
 
+/* Whole cycles of the same CPU algorithm, reference semantics (eps 0, overlap cutoff, elastic
+ * collisions): per cycle the worker pool's Compute phase on AoS bodies (force loop, then the
+ * detection loop keeping each overlapping Elastic/Fragment pair once), then the serial
+ * resolution of the pairs in canonical order (or_resolve's arithmetic) and the serial Update
+ * loop (body.go:114-139); computation-runner.go:294-320.  The state is advanced in place and
+ * is bit-identical to or_step with eps 0, cutoff and collisions on.  Returns the elapsed wall
+ * seconds of the nsteps cycles (negative on allocation failure). */
+double or_bench_cycles(or_state *s, double G, double dt, double R, int nsteps, int nthreads,
+                       or_stats *st);
+
 /* Barnes-Hut octree of cmd/body/octree.go with Build wired into BuildTree (nb_oracle_bh.c):
  * the reference's tree and traversal op by op, plus a depth limit (a leaf at max_depth keeps
  * further bodies as a bucket).  Pinned by: theta = 0 against the softened direct sum (every

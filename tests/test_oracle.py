@@ -314,3 +314,24 @@ def test_workpool_matches_force_and_counts_hits(oracle):
     np.testing.assert_array_equal(f, oracle.force_ref(st, G_REF))
     assert hits == 2 * len(oracle.detect(st)) + 500
     assert el > 0
+
+
+@pytest.mark.parametrize("threads", [1, 7])
+def test_bench_cycles_equals_the_serial_cycle(oracle, threads):
+    """The whole-cycle worker-pool baseline advances the state exactly as or_step does (reference
+    semantics, collisions on), so the timed CPU work is the reference's cycle.  The dense scene
+    makes collision pairs in most cycles; a zero-mass body turns NaN and is removed."""
+    b = gen.uniform_sphere(600, seed=3, radius=150.0)
+    b.m[5] = 0.0
+    a = oracle.State.from_bodies(b)
+    c = oracle.State.from_bodies(b)
+    ref = oracle.step(a, 6.673e-11, 0.0, True, True, 1e-3, 1.0, nsteps=12)
+    el, st = oracle.bench_cycles(c, 6.673e-11, 1e-3, 1.0, 12, threads)
+    assert el >= 0
+    assert ref["pairs_total"] > 50
+    for k in ("pairs_total", "pairs_last", "resolved_total", "removed_total"):
+        assert st[k] == ref[k], k
+    for k in ("x", "y", "z", "vx", "vy", "vz", "rest"):
+        np.testing.assert_array_equal(c.a[k], a.a[k])
+    np.testing.assert_array_equal(c.flags, a.flags)
+    assert not (c.flags[5] & oracle.F_EXISTS)
