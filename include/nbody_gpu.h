@@ -395,6 +395,28 @@ int nb_sur_get_stats(nb_ctx *ctx, nb_sur_stats *out);
 int nb_sur_save(nb_ctx *ctx, const char *path);
 int nb_sur_load(nb_ctx *ctx, const char *path);
 
+/* ---- force phase without device allocation ---------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  The force phase of
+ * nb_compute_forces on the softened FP32 path (eps > 0, no overlap cutoff, no collisions) for
+ * bodies the caller already holds on the device, with the kernels and operation order of a
+ * context created with the same G and eps and the shape {ipl 1, tile_j 512, jsplit min(4, tiles),
+ * block 256, variant 2}: the forces are bit-identical to that context's.
+ *   x, y, z, m: n FP64 device arrays; flags: n device bytes (NB_F_*; NULL = every body exists);
+ *   fx, fy, fz: n FP64 device outputs, Body.fx/fy/fz = m_i * G * sum_j m_j d / (d^2+eps^2)^1.5;
+ *   stream: a hipStream_t (NULL = the null stream).
+ * Asynchronous: kernels are enqueued on `stream` and the call returns without synchronising; no
+ * hipMalloc, hipFree or other allocating routine is called.  The intermediate source stream and
+ * per-slice partials live in a fixed device array of 28 MiB per device (held from library load
+ * until the process ends), which limits n to NB_NOALLOC_MAX_BODIES; larger n returns
+ * NB_E_INVALID.  The first call on a device resolves that array's address (make it before any
+ * stream capture).  Calls on one device share the array: calls that overlap on one device race.
+ * Errors: NB_E_INVALID / NB_E_HIP with the message in nb_last_error(NULL). */
+#define NB_NOALLOC_MAX_BODIES 262144
+int nb_forces_noalloc(int32_t device, int64_t n, const double *x, const double *y,
+                      const double *z, const double *m, const uint8_t *flags, double G,
+                      double eps, double *fx, double *fy, double *fz, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

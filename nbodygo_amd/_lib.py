@@ -225,7 +225,11 @@ SIGNATURES = [
     ("nb_sur_get_stats", C.c_int, [_ctx, C.POINTER(NbSurStats)]),
     ("nb_sur_save", C.c_int, [_ctx, C.c_char_p]),
     ("nb_sur_load", C.c_int, [_ctx, C.c_char_p]),
+    ("nb_forces_noalloc", C.c_int,
+     [C.c_int32, C.c_int64] + [C.c_void_p] * 5 + [C.c_double, C.c_double] + [C.c_void_p] * 4),
 ]
+
+NB_NOALLOC_MAX_BODIES = 262144
 
 
 def load_library(path=LIB_PATH):

@@ -33,6 +33,8 @@ static int fail(Ctx *c, int code, const std::string &msg) {
 
 int fail_msg(Ctx *c, int code, const char *msg) { return fail(c, code, msg); }
 
+void set_global_error(const char *msg) { g_create_err = msg; }
+
 template <class T>
 static hipError_t dalloc(T **p, size_t count) {
   if (*p) {

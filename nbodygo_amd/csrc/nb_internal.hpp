@@ -220,6 +220,7 @@ struct ForceArgs {
 hipError_t launch_force_f32(const ForceShape &s, const ForceArgs &a, hipStream_t st);
 ForceArgs make_force_args(Ctx *c);
 int fail_msg(Ctx *c, int code, const char *msg);
+void set_global_error(const char *msg);  // the message nb_last_error(NULL) returns
 void sur_free(Sur *s);
 int sur_force_phase(Ctx *c, bool timed, int ev_slot);
 void bh_free(Bh *b);
