@@ -70,6 +70,8 @@ def lib():
                                        C.POINTER(C.c_int64), C.c_int64, _dp, _dp, _dp]
         L.or_detect.argtypes = [sp, C.POINTER(C.c_int32), C.c_int64]
         L.or_detect.restype = C.c_int64
+        L.or_detect_grid.argtypes = [sp, C.POINTER(C.c_int32), C.c_int64, C.c_int]
+        L.or_detect_grid.restype = C.c_int64
         L.or_elastic.argtypes = [sp, C.c_int64, C.c_int64, _dp]
         L.or_resolve.argtypes = [sp, C.POINTER(C.c_int32), C.c_int64, C.POINTER(OrStats)]
         L.or_update.argtypes = [sp, _dp, _dp, _dp, C.c_double, C.c_double, C.POINTER(OrStats)]
@@ -167,6 +169,16 @@ def detect(st):
     n = lib().or_detect(C.byref(st.s), None, 0)
     out = np.zeros((max(n, 1), 2), np.int32)
     lib().or_detect(C.byref(st.s), out.ctypes.data_as(C.POINTER(C.c_int32)), n)
+    return out[:n]
+
+
+def detect_grid(st, nthreads=8):
+    """or_detect_grid: the canonical pair list by cell lists (large N)."""
+    n = lib().or_detect_grid(C.byref(st.s), None, 0, nthreads)
+    if n < 0:
+        raise RuntimeError("or_detect_grid failed %d" % n)
+    out = np.zeros((max(n, 1), 2), np.int32)
+    lib().or_detect_grid(C.byref(st.s), out.ctypes.data_as(C.POINTER(C.c_int32)), n, nthreads)
     return out[:n]
 
 

@@ -212,6 +212,150 @@ int64_t or_detect(const or_state *s, int32_t *pairs, int64_t cap) {
   return np;
 }
 
+/* Cell-list form of or_detect for large N: bodies are binned on a cubic grid whose cells are at
+ * least as wide as the largest sum of two radii, so every overlapping pair lies in the same or
+ * adjacent cells; each candidate is then decided by the same Collided test.  The pair list is
+ * the same canonical list (pinned against or_detect on the CPU). */
+typedef struct {
+  uint64_t key;
+  int64_t idx;
+} cell_entry;
+
+static int cell_cmp(const void *a, const void *b) {
+  const cell_entry *x = (const cell_entry *)a, *y = (const cell_entry *)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+typedef struct {
+  const or_state *s;
+  const cell_entry *ent;
+  int64_t nent;
+  double lo[3], h;
+  int64_t *cnt;     /* per body: pairs with a higher index */
+  int64_t *off;     /* per body: first slot in out */
+  int32_t *out;     /* 2 ints per pair, or NULL for the counting pass */
+} grid_ctx;
+
+static uint64_t cell_key(int64_t ix, int64_t iy, int64_t iz) {
+  return ((uint64_t)ix << 42) | ((uint64_t)iy << 21) | (uint64_t)iz;
+}
+
+static int64_t lower_key(const cell_entry *e, int64_t n, uint64_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) / 2;
+    if (e[mid].key < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+static int pair_cmp_j(const void *a, const void *b) {
+  const int32_t *x = (const int32_t *)a, *y = (const int32_t *)b;
+  return (x[1] > y[1]) - (x[1] < y[1]);
+}
+
+static void grid_range(void *c, int64_t lo_i, int64_t hi_i) {
+  grid_ctx *g = (grid_ctx *)c;
+  const or_state *s = g->s;
+  for (int64_t i = lo_i; i < hi_i; i++) {
+    if (!active(s, i) || !can_collide(s->beh[i])) continue;
+    const int64_t cx = (int64_t)floor((s->x[i] - g->lo[0]) / g->h);
+    const int64_t cy = (int64_t)floor((s->y[i] - g->lo[1]) / g->h);
+    const int64_t cz = (int64_t)floor((s->z[i] - g->lo[2]) / g->h);
+    int64_t k = 0, base = g->out ? g->off[i] : 0;
+    for (int64_t dx = -1; dx <= 1; dx++)
+      for (int64_t dy = -1; dy <= 1; dy++)
+        for (int64_t dz = -1; dz <= 1; dz++) {
+          const int64_t ax = cx + dx, ay = cy + dy, az = cz + dz;
+          if (ax < 0 || ay < 0 || az < 0 || ax >= (1 << 21) || ay >= (1 << 21) || az >= (1 << 21))
+            continue;
+          const uint64_t key = cell_key(ax, ay, az);
+          for (int64_t q = lower_key(g->ent, g->nent, key); q < g->nent && g->ent[q].key == key;
+               q++) {
+            const int64_t j = g->ent[q].idx;
+            if (j <= i || !collided(s, i, j)) continue;
+            if (g->out) {
+              g->out[2 * (base + k)] = (int32_t)i;
+              g->out[2 * (base + k) + 1] = (int32_t)j;
+            }
+            k++;
+          }
+        }
+    if (g->out) qsort(g->out + 2 * base, (size_t)k, 2 * sizeof(int32_t), pair_cmp_j);
+    else g->cnt[i] = k;
+  }
+}
+
+int64_t or_detect_grid(const or_state *s, int32_t *pairs, int64_t cap, int nthreads) {
+  const int64_t n = s->n;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double rmax = 0;
+  int64_t na = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (!active(s, i) || !can_collide(s->beh[i])) continue;
+    const double p[3] = {s->x[i], s->y[i], s->z[i]};
+    if (!(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]))) return -2;
+    for (int d = 0; d < 3; d++) {
+      if (p[d] < lo[d]) lo[d] = p[d];
+      if (p[d] > hi[d]) hi[d] = p[d];
+    }
+    if (s->r[i] > rmax) rmax = s->r[i];
+    na++;
+  }
+  if (na < 2) return 0;
+  /* cells at least 2 rmax wide (every overlap spans at most one cell boundary per axis), with
+   * the 1.0000001 margin covering the rounding of the cell index computation, and few enough
+   * that an index fits 21 bits */
+  double span = 0;
+  for (int d = 0; d < 3; d++)
+    if (hi[d] - lo[d] > span) span = hi[d] - lo[d];
+  double h = 2.0 * rmax * 1.0000001;
+  if (h < span / (double)((1 << 20))) h = span / (double)((1 << 20));
+  if (!(h > 0)) h = 1.0;
+  cell_entry *ent = (cell_entry *)malloc(sizeof(cell_entry) * (size_t)na);
+  int64_t *cnt = (int64_t *)calloc((size_t)n, sizeof(int64_t));
+  int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+  if (!ent || !cnt || !off) {
+    free(ent);
+    free(cnt);
+    free(off);
+    return -3;
+  }
+  int64_t e = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (!active(s, i) || !can_collide(s->beh[i])) continue;
+    ent[e].key = cell_key((int64_t)floor((s->x[i] - lo[0]) / h),
+                          (int64_t)floor((s->y[i] - lo[1]) / h),
+                          (int64_t)floor((s->z[i] - lo[2]) / h));
+    ent[e].idx = i;
+    e++;
+  }
+  qsort(ent, (size_t)na, sizeof(cell_entry), cell_cmp);
+  grid_ctx g = {s, ent, na, {lo[0], lo[1], lo[2]}, h, cnt, off, NULL};
+  parallel_for(n, nthreads, grid_range, &g);
+  off[0] = 0;
+  for (int64_t i = 0; i < n; i++) off[i + 1] = off[i] + cnt[i];
+  const int64_t np = off[n];
+  int32_t *all = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(np > 0 ? np : 1));
+  if (!all) {
+    free(ent);
+    free(cnt);
+    free(off);
+    return -3;
+  }
+  g.out = all;
+  parallel_for(n, nthreads, grid_range, &g);
+  const int64_t c = np < cap ? np : cap;
+  if (c > 0) memcpy(pairs, all, sizeof(int32_t) * 2 * (size_t)c);
+  free(all);
+  free(ent);
+  free(cnt);
+  free(off);
+  return np;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Elastic collision: calcElasticCollision, collisioncalc.go:42-186 (coll3d)                  */
 /* ------------------------------------------------------------------------------------------ */

@@ -92,6 +92,12 @@ void or_force_soft_ld(const or_state *s, double G, double eps, int cutoff, const
  * Returns the number of pairs found (may exceed cap; only cap are written). */
 int64_t or_detect(const or_state *s, int32_t *pairs, int64_t cap);
 
+/* The same canonical pair list by cell lists, for large N: cells at least twice the largest
+ * radius wide, each candidate decided by the same Collided test.  Returns the pair count (only
+ * cap pairs are written), -2 for a non-finite position of a collidable body, -3 on allocation
+ * failure. */
+int64_t or_detect_grid(const or_state *s, int32_t *pairs, int64_t cap, int nthreads);
+
 /* calcElasticCollision (cmd/body/collisioncalc.go:42-186) for body b1 (index i) and b2 (j).
  * out[0..8] = vx1,vy1,vz1,vx2,vy2,vz2,vx_cm,vy_cm,vz_cm.  Returns 1 if "collided". */
 int or_elastic(const or_state *s, int64_t i, int64_t j, double *out);

@@ -151,6 +151,42 @@ def test_detect_matches_brute_force_with_edge_cases(oracle):
     assert (0, 1) in pairs and (2, 3) in pairs
 
 
+@pytest.mark.parametrize("scene", ["edge_cases", "two_spheres", "mixed_radii", "cell_faces"])
+def test_detect_grid_matches_brute_force(oracle, scene):
+    """The cell-list detector gives the brute-force detector's canonical pair list: boundary
+    geometries (d == R1 + R2, coincident), inactive / None / Fragment bodies, two clusters far
+    apart (cells of very different occupancy), radii of widely different size, and pairs placed
+    across cell faces at exactly the cell width."""
+    if scene == "edge_cases":
+        b = gen.uniform_sphere(600, seed=4, radius_k=0.12)
+        b.x[:4] = [0.0, 3.0, 0.0, 0.0]
+        b.y[:4] = [0.0, 4.0, 0.0, 0.0]
+        b.z[:4] = [5000.0, 5000.0, 6000.0, 6000.0]
+        b.r[:4] = [2.5, 2.5, 1.0, 1.0]
+        b.flags[10] = 0
+        b.flags[11] |= 2
+        b.beh[12] = 0
+        b.beh[13] = 3
+    elif scene == "two_spheres":
+        b = gen.two_plummer(1500, seed=5, radius_k=0.6)
+    elif scene == "mixed_radii":
+        b = gen.uniform_sphere(2000, seed=6, radius_k=0.05)
+        b.r[::97] *= 20.0
+    else:
+        rng = np.random.default_rng(7)
+        n = 1200
+        pos = rng.uniform(0.0, 8.0, (n, 3)).round(0)  # a lattice of unit spacing
+        pos[n // 2:] += [1.0, 0.0, 0.0]               # duplicated one cell over: d == 1 == 2 r
+        from nbodygo_amd import Bodies
+        b = Bodies(pos[:, 0], pos[:, 1], pos[:, 2], *np.zeros((3, n)), np.ones(n),
+                   np.full(n, 0.5))
+    st = oracle.State.from_bodies(b)
+    exp = oracle.detect(st)
+    assert len(exp) > 20
+    for threads in (1, 5):
+        np.testing.assert_array_equal(oracle.detect_grid(st, nthreads=threads), exp)
+
+
 def golden_state(oracle, c):
     x1, x2, v1, v2 = c["x1"], c["x2"], c["v1"], c["v2"]
     st = oracle.State([x1[0], x2[0]], [x1[1], x2[1]], [x1[2], x2[2]], [v1[0], v2[0]],
