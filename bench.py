@@ -36,6 +36,12 @@ BOUND_FP32 = "compute: FP32 vector ALU (v_fma_f32, v_rsq_f32), peak 157.3 TFLOP/
 BOUND_FP64 = "compute: FP64 vector ALU (divisions, square roots), peak 78.6 TFLOP/s"
 FLOPS_PER_INTERACTION = 20
 ROOT = os.path.dirname(os.path.abspath(__file__))
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print("bench [%6.1f s] %s" % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
 
 WORKLOADS = {
     "plummer262k": dict(n=262144, desc="Plummer sphere N=262144 (a=1, M=1, G=1), eps=0.01, "
@@ -58,8 +64,9 @@ WORKLOADS = {
                                           "energy and momentum drift from FP64 diagnostics"),
     "barneshut262k": dict(n=262144, desc="Plummer sphere N=262144 (a=1, M=1, G=1), dt=1e-3, "
                                          "collisions off: the reference's Barnes-Hut octree "
-                                         "(theta 0.3, softening 1e-4, Build wired in) in FP64 "
-                                         "against the FP64 direct sum with the same softening"),
+                                         "(theta 0.3, softening 1e-4, Build wired in), exact "
+                                         "FP64 walk and fast group walk, against the FP64 "
+                                         "direct sum with the same softening"),
     "lifecycle1m": dict(n=1048576, desc="Plummer sphere N=1,048,576, FP32 path: one delete "
                                        "request of 1 % of the bodies (processDeletes' walk), "
                                        "1,000 queued adds and Cycle, on the device, against "
@@ -133,7 +140,7 @@ def measured_traffic(n, shape):
     """HBM bytes per force launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     (calibrated against a copy kernel of known size), when they were taken at this body count
     and kernel shape; None otherwise."""
-    name = "force_plummer%dk_traffic.json" % (n // 1024)
+    name = "force_plummer%dk_traffic.json" % (n // 1000)
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
@@ -364,45 +371,53 @@ FLOPS_PER_BH_TERM = 23   # 3 sub, 3 mul + 3 add (softened r^2), sqrt, 2h and /di
 
 def run_barneshut(a, w):
     """The reference's own far-field design: the octree with Build wired in, FP64, against the
-    FP64 direct sum with the same softening at the same N."""
+    FP64 direct sum with the same softening at the same N; exact mode (the reference's walk and
+    arithmetic per body) and fast mode (one walk per 64-body group, FP32 terms)."""
     from nbodygo_amd import NBodyGPU, gen
     n = w["n"]
     b = gen.plummer(n, seed=2024)
-    g = NBodyGPU(G=1.0, eps=1e-4, overlap_cutoff=False, precision="fp64", collisions=False)
-    g.load(b)
-    g.bh_enable()
-    s, el = timed_run(g, b, a.steps, a.warmup)
-    st = g.bh_stats()
-    ms_step = el / a.steps * 1e3
-    launch_ms = s["force_ms"] / max(s["force_launches"], 1)
-    terms = st["interactions"]
-    achieved = FLOPS_PER_BH_TERM * terms / (launch_ms * 1e-3)
-    # force error: the octree force against FP64 direct-sum accelerations of a sample
-    g.load(b)
     idx = np.sort(np.random.default_rng(0).choice(n, 2048, replace=False))
-    f = g.forces()[idx] / b.m[idx, None]
-    ref = g.accel_fp64(idx)
-    err = float(np.linalg.norm(f - ref) / np.linalg.norm(ref))
-    g.close()
+    modes = {}
+    for mode in ("exact", "fast"):
+        g = NBodyGPU(G=1.0, eps=1e-4, overlap_cutoff=False, precision="fp64", collisions=False)
+        g.load(b)
+        g.bh_enable(mode=mode)
+        s, el = timed_run(g, b, a.steps, a.warmup)
+        st = g.bh_stats()
+        ms_step = el / a.steps * 1e3
+        launch_ms = s["force_ms"] / max(s["force_launches"], 1)
+        g.load(b)
+        f = g.forces()[idx] / b.m[idx, None]
+        ref = g.accel_fp64(idx)
+        g.close()
+        modes[mode] = {"ms_per_step": ms_step, "force_ms": launch_ms,
+                       "terms_per_step": st["interactions"], "terms_per_body": st["interactions"] / n,
+                       "G_terms_per_s": st["interactions"] / (launch_ms * 1e-3) / 1e9,
+                       "force_error_vs_fp64_direct": error_summary(f, ref),
+                       "cells": st["nodes"], "depth_max": st["depth_max"]}
+        log("barneshut %s: %.3f ms/step" % (mode, ms_step))
     h = NBodyGPU(G=1.0, eps=1e-4, overlap_cutoff=False, precision="fp64", collisions=False)
     sd, eld = timed_run(h, b, 2, 1)
     h.close()
     direct_ms = eld / 2 * 1e3
+    ex, fa = modes["exact"], modes["fast"]
+    achieved = FLOPS_PER_BH_TERM * ex["terms_per_step"] / (ex["force_ms"] * 1e-3)
+    for m in modes.values():
+        m["speedup_vs_fp64_direct"] = direct_ms / m["ms_per_step"]
+    fa["speedup_vs_exact"] = ex["ms_per_step"] / fa["ms_per_step"]
     print(json.dumps({
-        "metric": METRIC, "value": n * (n - 1) / (ms_step * 1e-3) / 1e9,
-        "unit": "G interactions/s (N(N-1) per step, Barnes-Hut)", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
-        "steps_per_s": 1e3 / ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp64", "data": "synthetic (seeded Plummer sphere)",
+        "metric": METRIC, "value": n * (n - 1) / (ex["ms_per_step"] * 1e-3) / 1e9,
+        "unit": "G interactions/s (N(N-1) per step, Barnes-Hut exact mode)", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ex["ms_per_step"],
+        "steps_per_s": 1e3 / ex["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64 (exact mode); fp32 terms (fast mode)",
+        "data": "synthetic (seeded Plummer sphere)",
         "config": {"workload": w["desc"], "n": n, "theta": 0.3, "softening": 1e-4},
-        "tree": {"terms_per_step": terms, "terms_per_body": terms / n, "cells": st["nodes"],
-                 "depth_max": st["depth_max"], "G_terms_per_s": terms / (launch_ms * 1e-3) / 1e9},
-        "force_error_vs_fp64_direct": err,
-        "fp64_direct": {"ms_per_step": direct_ms, "speedup_of_tree": direct_ms / ms_step},
+        "exact": ex, "fast": fa, "fp64_direct": {"ms_per_step": direct_ms},
         "roofline": {"bound": "mfma", "bound_detail": BOUND_FP64, "achieved": achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                     "traffic": None, "kernel": "k_bh_force + k_bh_sun (tree walk)",
-                     "flops_per_term": FLOPS_PER_BH_TERM, "avg_launch_ms": launch_ms},
+                     "traffic": None, "kernel": "k_bh_force + k_bh_sun (exact tree walk)",
+                     "flops_per_term": FLOPS_PER_BH_TERM, "avg_launch_ms": ex["force_ms"]},
     }))
 
 
@@ -568,6 +583,7 @@ def main():
         return
     n = w["n"]
     p = plain_leg(n, a.steps, a.warmup, pinned, tune)
+    log("plain N=%d: %.3f ms/step, shape %s" % (n, p["ms_per_step"], p["kernel_shape"]))
     out = {
         "metric": METRIC, "value": p["value"], "unit": p["unit"], "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": p["ms_per_step"],
@@ -583,17 +599,22 @@ def main():
     if a.workload == "plummer262k" and not a.headline_only:
         if not a.no_surrogate:
             out["surrogate"] = surrogate_leg(a, pinned, plain=p)
+            log("surrogate: %.3f ms/step" % out["surrogate"]["ms_per_step"])
         if not a.no_extras:
             c2 = plain_leg(65536, a.steps, a.warmup, pinned, tune)
             if not a.no_elastic:
                 c2["elastic"] = elastic_leg(65536, 1e-4, a.steps, a.warmup, pinned, tune)
                 c2["elastic_dense"] = elastic_leg(65536, 5e-3, a.steps, a.warmup, pinned, tune)
             out["plummer65k"] = c2
+            log("plummer65k: %.3f ms/step" % c2["ms_per_step"])
             out["batch512"] = batch_leg(a.steps, a.warmup, tune)
+            log("batch512: %.3f ms/step" % out["batch512"]["ms_per_step"])
             out["collide2x524k"] = collide_leg(a, pinned, a.c5_steps or 20)
+            log("collide2x524k: %.1f x" % out["collide2x524k"]["speedup_vs_plain"])
     if not (a.no_cpu or a.headline_only):
         out["cpu_baseline"] = cpu_baseline(gen.plummer(n, seed=2024), a.cpu_seconds)
         out["cpu_config1"] = cpu_config1()
+        log("cpu legs done")
     print(json.dumps(out))
 
 

@@ -79,7 +79,10 @@ typedef struct nb_params {
                              doFragment / fragment, fragcalc.go:54-117); 0 = elastic collisions
                              only (Subsume and None bodies make no pairs, shouldFragment is only
                              counted) */
-  int32_t reserved[6];
+  int32_t resolve;        /* collision resolution: 0 = pairs that share no body resolve together
+                             (events off), 1 = one pair at a time on a single lane; both walk the
+                             pairs in canonical order and give bit-identical states */
+  int32_t reserved[5];
 } nb_params;
 
 typedef struct nb_stats {
@@ -100,6 +103,7 @@ typedef struct nb_stats {
   double wall_ms;            /* host wall time of the call */
   double force_ms;           /* HIP-event time of the force kernels (timing enabled only) */
   int64_t force_launches;    /* force kernel launches timed */
+  double resolve_ms;         /* HIP-event time of the collision resolution (timing enabled) */
 } nb_stats;
 
 typedef struct nb_diag {
@@ -260,7 +264,13 @@ typedef struct nb_bh_params {
                         root cell / 2^max_depth share it (the reference recurses without end
                         for coincident bodies).  Default 21. */
   int32_t group;     /* bodies per wavefront in the tree walk: 16, 32 or 64; 0 = 64 */
-  int32_t reserved[6];
+  int32_t mode;      /* 0 = exact: each body walks the tree with the reference's test and FP64
+                        terms in the reference's order; 1 = fast: the 64 bodies of a wavefront
+                        (neighbours in key order) share one walk that opens every node any of
+                        them would open (theta against the distance to the group's bounding box),
+                        and the terms are summed in FP32 (the body's own term excluded); forces
+                        then differ from exact mode by the FP32 and opening differences */
+  int32_t reserved[5];
 } nb_bh_params;
 
 typedef struct nb_bh_stats {
@@ -311,11 +321,17 @@ int nb_batch_forces(nb_batch *b, double *fx, double *fy, double *fz);
 int nb_batch_store(nb_batch *b, double *x, double *y, double *z, double *vx, double *vy,
                    double *vz, uint8_t *flags);
 
-/* Target-block width (targets per lane: 1, 2 or 4) for the size buckets N < 1024, < 4096,
- * >= 4096.  nb_batch_autotune times each choice per bucket on the loaded simulations and keeps
- * the fastest. */
+/* Kernel shape of the batched force.  Work items are blocks of 64 x IPL targets of one
+ * simulation; `waves` (1, 2 or 4) wavefronts share an item, splitting its sources; each wave
+ * consumes its sources in Kahan tiles of `tile` (64, 128 or 256) sources.  IPL and tile are chosen
+ * per size bucket (N < 1024, < 4096, >= 4096), waves for the whole launch.
+ * nb_batch_autotune times every (IPL, tile) per bucket for every waves choice on the loaded
+ * simulations and keeps the fastest; ipl3 (may be NULL) receives the per-bucket IPL.
+ * Tuning words: cfg7 = {waves, ipl0, tile0, ipl1, tile1, ipl2, tile2}. */
 int nb_batch_get_shapes(nb_batch *b, int32_t *ipl3);
 int nb_batch_set_shapes(nb_batch *b, const int32_t *ipl3);
+int nb_batch_get_tuning(nb_batch *b, int32_t *cfg7);
+int nb_batch_set_tuning(nb_batch *b, const int32_t *cfg7);
 int nb_batch_autotune(nb_batch *b, int32_t reps, int32_t *ipl3);
 
 /* ---- small MLP on the matrix cores (the learned part of the far-field surrogate) ----------- */
@@ -349,7 +365,7 @@ int nb_mlp_train(nb_mlp *m, const float *x, const float *t, int64_t rows, int32_
  * overlap cutoff), nb_step and nb_compute_forces replace the O(N^2) force with an exact near
  * field over adjacent cells of a 32^3 grid in stretched coordinates u = d / (|d| + r0) about the
  * centre of mass, per-body quadrupoles of the cells at Chebyshev distance 2..shell, a
- * second-order Taylor far expansion, and an MLP residual; every check_every steps a sampled
+ * third-order Taylor far expansion, and an MLP residual; every check_every steps a sampled
  * direct-sum error above tol makes the step fall back to the direct sum.  With collisions on,
  * detection runs in the near field (exact FP64 test, the same pairs as the direct path); this
  * needs every radius <= r0 / grid, so that each overlapping pair lies in adjacent cells.
@@ -362,9 +378,14 @@ typedef struct nb_sur_params {
   int32_t check_every;   /* steps between sampled error checks; 0 = never (1) */
   int32_t check_samples; /* bodies per check (256) */
   int32_t pad0;
-  double tol;            /* sampled relative error above which the step is redone (1e-3) */
-  uint64_t seed;
-  int32_t reserved[8];
+  double tol;            /* sampled relative error ||da|| / ||a|| above which the step is redone
+                            (1e-3) */
+  uint64_t seed;         /* training shuffles, initial weights and the check's sample draws */
+  double tol_body;       /* per-body bar: the step is also redone when more than a fraction
+                            1 - body_quantile of the samples have |da_i| / |a_i| above it (0 =
+                            off; 0) */
+  double body_quantile;  /* (0.99) */
+  int32_t reserved[4];
 } nb_sur_params;
 
 typedef struct nb_sur_stats {
@@ -379,6 +400,7 @@ typedef struct nb_sur_stats {
   double r0;               /* stretch radius in use */
   int32_t trained;
   int32_t pad1;
+  double last_body_error;  /* the body_quantile quantile of the per-body errors at the last check */
 } nb_sur_stats;
 
 int nb_sur_params_init(nb_sur_params *p);
@@ -388,8 +410,10 @@ int nb_sur_disable(nb_ctx *ctx);
 int nb_sur_collect(nb_ctx *ctx, int64_t samples, uint64_t seed);
 /* Fit the MLP on the collected rows with the device Adam (steps minibatches of batch rows). */
 int nb_sur_fit(nb_ctx *ctx, int32_t steps, int32_t batch, float lr, float *loss);
-/* Sampled relative force error of the surrogate on the loaded state, with (1) or without (0) the
- * learned residual. */
+/* Sampled relative force error ||da|| / ||a|| of the surrogate on the loaded state, with (1) or
+ * without (0) the learned residual: one body from each of `samples` equal strata of the
+ * cell-sorted order, the same draw at every call (so settings compare on the same bodies; the
+ * checks inside steps draw afresh each time). */
 int nb_sur_error(nb_ctx *ctx, int32_t samples, int32_t use_mlp, double *err);
 int nb_sur_get_stats(nb_ctx *ctx, nb_sur_stats *out);
 int nb_sur_save(nb_ctx *ctx, const char *path);

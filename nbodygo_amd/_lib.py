@@ -46,7 +46,8 @@ class NbParams(C.Structure):
         ("block", C.c_int32),
         ("variant", C.c_int32),
         ("events", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("resolve", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -67,6 +68,7 @@ class NbStats(C.Structure):
         ("wall_ms", C.c_double),
         ("force_ms", C.c_double),
         ("force_launches", C.c_int64),
+        ("resolve_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -94,7 +96,9 @@ class NbSurParams(C.Structure):
         ("pad0", C.c_int32),
         ("tol", C.c_double),
         ("seed", C.c_uint64),
-        ("reserved", C.c_int32 * 8),
+        ("tol_body", C.c_double),
+        ("body_quantile", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -111,6 +115,7 @@ class NbSurStats(C.Structure):
         ("r0", C.c_double),
         ("trained", C.c_int32),
         ("pad1", C.c_int32),
+        ("last_body_error", C.c_double),
     ]
 
     def as_dict(self):
@@ -119,7 +124,7 @@ class NbSurStats(C.Structure):
 
 class NbBhParams(C.Structure):
     _fields_ = [("theta", C.c_double), ("softening", C.c_double), ("max_depth", C.c_int32),
-                ("group", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("group", C.c_int32), ("mode", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class NbBhStats(C.Structure):
@@ -204,6 +209,8 @@ SIGNATURES = [
     ("nb_batch_get_shapes", C.c_int, [_ctx, _i32p]),
     ("nb_batch_set_shapes", C.c_int, [_ctx, _i32p]),
     ("nb_batch_autotune", C.c_int, [_ctx, C.c_int32, _i32p]),
+    ("nb_batch_get_tuning", C.c_int, [_ctx, _i32p]),
+    ("nb_batch_set_tuning", C.c_int, [_ctx, _i32p]),
     ("nb_mlp_create", C.c_int,
      [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_ctx)]),
     ("nb_mlp_destroy", None, [_ctx]),

@@ -69,7 +69,7 @@ static void free_all(Ctx *c) {
   void *ptrs[] = {c->x,    c->y,      c->z,     c->vx,   c->vy,        c->vz,   c->m,
                   c->r,    c->rest,   c->frag,  c->beh,  c->flags,     c->collided,
                   c->js,   c->rmax_tile, c->acc, c->fcnt, c->flist, c->recs, c->pairs, c->ctr,
-                  c->diag};
+                  c->diag, c->pairs_tmp, c->pairs_sw, c->res_words};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->x = c->y = c->z = c->vx = c->vy = c->vz = c->m = c->r = c->rest = c->frag = nullptr;
@@ -80,7 +80,8 @@ static void free_all(Ctx *c) {
   c->acc = nullptr;
   c->fcnt = c->flist = nullptr;
   c->recs = nullptr;
-  c->pairs = nullptr;
+  c->pairs = c->pairs_tmp = c->pairs_sw = nullptr;
+  c->res_words = nullptr;
   c->ctr = nullptr;
   c->diag = nullptr;
 }
@@ -163,6 +164,9 @@ static int size_pairs(Ctx *c, int64_t n) {
   while (pow2 < pcap) pow2 <<= 1;
   if (pow2 > c->pair_cap_pow2) {
     NB_HIP(c, dalloc(&c->pairs, (size_t)pow2));
+    NB_HIP(c, dalloc(&c->pairs_tmp, (size_t)pow2));
+    NB_HIP(c, dalloc(&c->pairs_sw, (size_t)pow2));
+    NB_HIP(c, dalloc(&c->res_words, (size_t)pow2 * 5));
     c->pair_cap_pow2 = pow2;
   }
   c->pair_cap = pcap;
@@ -249,7 +253,12 @@ static int force_phase(Ctx *c, bool timed, int ev_slot) {
       NB_HIP(c, launch_fixup(f, c->stream));
     }
   }
-  if (c->p.collisions) NB_HIP(c, launch_sort_pairs(c->pairs, c->pair_cap, c->ctr, c->stream));
+  if (c->p.collisions) return sort_pairs(c);
+  return NB_OK;
+}
+
+int sort_pairs(Ctx *c) {
+  NB_HIP(c, launch_sort_pairs(c->pairs, c->pairs_tmp, c->pair_cap, c->ctr, c->stream));
   return NB_OK;
 }
 
@@ -307,6 +316,10 @@ int nb_create(const nb_params *p, nb_ctx **out) {
     g_create_err = "nb_create: unknown precision mode";
     return NB_E_INVALID;
   }
+  if (p->resolve != 0 && p->resolve != 1) {
+    g_create_err = "nb_create: resolve must be 0 or 1";
+    return NB_E_INVALID;
+  }
   if (p->precision == NB_PREC_FP32 && p->eps > 0.0 && (float)(p->eps * p->eps) < 1e-30f) {
     g_create_err = "nb_create: eps^2 underflows FP32; use eps = 0 or NB_PREC_FP64";
     return NB_E_INVALID;
@@ -353,6 +366,7 @@ void nb_destroy(nb_ctx *c) {
   free_all(c);
   if (c->small) (void)hipFree(c->small);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->ev_res) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -462,6 +476,11 @@ int nb_step(nb_ctx *c, double dt, double R, int32_t nsteps, nb_stats *out) {
       NB_HIP(c, hipEventCreate(&e));
       c->ev.push_back(e);
     }
+    while (c->p.collisions && (int)c->ev_res.size() < 2 * nsteps) {
+      hipEvent_t e;
+      NB_HIP(c, hipEventCreate(&e));
+      c->ev_res.push_back(e);
+    }
   }
   const int64_t na0 = c->n_active;
   if (c->p.events) {
@@ -491,7 +510,16 @@ int nb_step(nb_ctx *c, double dt, double R, int32_t nsteps, nb_stats *out) {
       ra.acc = c->acc;
       ra.split = split_now(c);
       ra.npad = (int)c->npad;
+      ra.pairs_sw = c->pairs_sw;
+      ra.sort_tmp = c->pairs_tmp;
+      ra.succ = c->res_words;
+      ra.pending = c->res_words + 2 * c->pair_cap_pow2;
+      ra.list_a = c->res_words + 3 * c->pair_cap_pow2;
+      ra.list_b = c->res_words + 4 * c->pair_cap_pow2;
+      ra.serial = c->p.resolve == 1 ? 1 : 0;
+      if (timed) NB_HIP(c, hipEventRecord(c->ev_res[2 * s], c->stream));
       NB_HIP(c, launch_resolve(ra, c->stream));
+      if (timed) NB_HIP(c, hipEventRecord(c->ev_res[2 * s + 1], c->stream));
     }
     UpdateArgs u;
     u.x = c->x; u.y = c->y; u.z = c->z; u.vx = c->vx; u.vy = c->vy; u.vz = c->vz;
@@ -516,12 +544,16 @@ int nb_step(nb_ctx *c, double dt, double R, int32_t nsteps, nb_stats *out) {
   NB_HIP(c, hipMemcpyAsync(&after, c->ctr, sizeof after, hipMemcpyDeviceToHost, c->stream));
   NB_HIP(c, hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::steady_clock::now();
-  double force_ms = 0;
+  double force_ms = 0, resolve_ms = 0;
   if (timed) {
     for (int s = 0; s < nsteps; s++) {
       float ms = 0;
       NB_HIP(c, hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]));
       force_ms += ms;
+      if (c->p.collisions) {
+        NB_HIP(c, hipEventElapsedTime(&ms, c->ev_res[2 * s], c->ev_res[2 * s + 1]));
+        resolve_ms += ms;
+      }
     }
   }
   const int64_t removed = (int64_t)(after.removed_total - before.removed_total) +
@@ -544,6 +576,7 @@ int nb_step(nb_ctx *c, double dt, double R, int32_t nsteps, nb_stats *out) {
     out->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     out->force_ms = force_ms;
     out->force_launches = timed ? nsteps : 0;
+    out->resolve_ms = resolve_ms;
   }
   if (after.pair_overflow)
     return fail(c, NB_E_PAIR_OVERFLOW,

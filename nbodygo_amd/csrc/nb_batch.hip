@@ -9,10 +9,12 @@
 // with far-away massless sentinels (flags = 0), so no 64-source tile straddles two simulations.
 // Work items are (simulation, block of 64*IPL targets), sorted longest first (cost = targets x
 // sources) and pulled from a device queue by a persistent grid: every workgroup takes items until
-// the queue is exhausted, so the grid drains on its own.  The W waves of a workgroup share the
-// item's targets and split its sources into W contiguous ranges; their Kahan-compensated partial
-// sums are combined through LDS in wave order, so results do not depend on scheduling.  The
-// target-block width (IPL) is chosen per size bucket by nb_batch_autotune.
+// the queue is exhausted, so the grid drains on its own.  The W waves of a workgroup (W = 1, 2 or
+// 4) share the item's targets and split its sources into W contiguous ranges; their
+// Kahan-compensated partial sums are combined through LDS in wave order, so results do not
+// depend on scheduling (with W = 1 each wave takes items of its own).  The target-block width
+// (IPL) and the Kahan tile (64, 128 or 256 sources) are chosen per size bucket, W per launch, by
+// nb_batch_autotune.
 //
 // One nb_batch_step call runs nsteps cycles as one hipGraph launch: per cycle a fill kernel resets
 // the queue (no memset nodes), then the force kernel, then the update kernel.
@@ -32,9 +34,9 @@
 namespace nb {
 
 constexpr int kSimPad = 64;
-constexpr int kBatchWaves = 4;        // waves per workgroup (source split)
 constexpr int kBatchMaxTargets = 256; // 64 lanes * IPL (IPL <= 4)
 constexpr int kBuckets = 3;           // size buckets for shape selection
+constexpr int kBatchMaxTile = 256;    // sources per Kahan tile (64, 128 or 256)
 
 struct BatchItem {
   int32_t t0;     // first target (global index)
@@ -42,7 +44,8 @@ struct BatchItem {
   int32_t s0;     // first source of the simulation (global index)
   int32_t sn;     // sources (padded simulation size, multiple of 64)
   int32_t ipl;    // 1, 2 or 4
-  int32_t pad[3];
+  int32_t tile;   // sources per Kahan tile: 64, 128 or 256
+  int32_t pad[2];
 };
 
 struct BatchQueue {
@@ -59,14 +62,21 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-template <int IPL>
+// One work item on the W waves of a workgroup: its 64 * IPL targets against the simulation's
+// sources, which the waves split into W contiguous ranges of TILE-source tiles (the last tile of
+// a range may be shorter: the range ends on a 64-source boundary).  Each wave stages its tile in
+// its own LDS buffer (one 16-byte load per lane per 64 sources), reads it back as wave-uniform
+// broadcasts, keeps plain FP32 partial sums within the tile and Kahan-adds them at the tile's
+// end.  With W > 1 the waves' FP64 results are added in wave order through LDS.
+template <int W, int IPL, int TILE>
 __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, double *acc,
                            int npad, float eps2_s, double (*lds)[3][kBatchMaxTargets],
-                           float4 (*tiles_lds)[64]) {
+                           float4 (*tiles_lds)[kBatchMaxTile]) {
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
+  const int wave = threadIdx.x >> 6;          // the workgroup's wave: its LDS tile
+  const int w = W == 1 ? 0 : wave;            // its share of the item's sources
   const float eps2 = to_vgpr(eps2_s);
-  float4 *tl = tiles_lds[w];
+  float4 *tl = tiles_lds[wave];
   float xi[IPL], yi[IPL], zi[IPL], sx[IPL], sy[IPL], sz[IPL], cx[IPL], cy[IPL], cz[IPL];
 #pragma unroll
   for (int k = 0; k < IPL; k++) {
@@ -78,19 +88,18 @@ __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, d
     zi[k] = b.z;
     sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0.f;
   }
-  // wave w takes tiles [w*tiles/W, (w+1)*tiles/W) of the simulation's 64-source tiles
-  const int tiles = it.sn / 64;
-  const int ta = w * tiles / kBatchWaves, tb = (w + 1) * tiles / kBatchWaves;
-  for (int t = ta; t < tb; t++) {
+  // wave w takes the 64-source blocks [w*blocks/W, (w+1)*blocks/W) of the simulation
+  const int blocks = it.sn / 64;
+  const int ja = (w * blocks / W) * 64, jb = ((w + 1) * blocks / W) * 64;
+  for (int t0 = ja; t0 < jb; t0 += TILE) {
+    const int cnt = min(TILE, jb - t0);  // a multiple of 64
     float px[IPL], py[IPL], pz[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; k++) px[k] = py[k] = pz[k] = 0.f;
-    // each wave stages its own 64-source tile: one 16-byte load per lane, then wave-uniform
-    // ds_read_b128 broadcasts (source operands in VGPRs, so the VALU ops can dual-issue)
-    tl[lane] = js[it.s0 + t * 64 + lane];
+    for (int q = lane; q < cnt; q += 64) tl[q] = js[it.s0 + t0 + q];
     wave_lds_sync();
 #pragma unroll 4
-    for (int jj = 0; jj < 64; jj++) {
+    for (int jj = 0; jj < cnt; jj++) {
       const float4 b = tl[jj];
       float dx[IPL], dy[IPL], dz[IPL], inv[IPL];
 #pragma unroll
@@ -122,6 +131,19 @@ __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, d
     }
     wave_lds_sync();  // all reads of this tile precede the next tile's writes
   }
+  if (W == 1) {
+#pragma unroll
+    for (int k = 0; k < IPL; k++) {
+      const int local = lane + 64 * k;
+      if (local < it.tn) {
+        const int i = it.t0 + local;
+        acc[i] = (double)sx[k] - (double)cx[k];
+        acc[npad + i] = (double)sy[k] - (double)cy[k];
+        acc[2 * (size_t)npad + i] = (double)sz[k] - (double)cz[k];
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < IPL; k++) {
     const int local = lane + 64 * k;
@@ -134,7 +156,7 @@ __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, d
     double f[3];
     for (int c = 0; c < 3; c++) {
       double v = lds[0][c][q];
-      for (int ww = 1; ww < kBatchWaves; ww++) v += lds[ww][c][q];
+      for (int ww = 1; ww < W; ww++) v += lds[ww][c][q];
       f[c] = v;
     }
     const int i = it.t0 + q;
@@ -144,25 +166,49 @@ __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, d
   }
 }
 
-__global__ __launch_bounds__(64 * kBatchWaves) void k_batch_force(
-    const BatchItem *__restrict__ items, int nitems, BatchQueue *queue,
-    const float4 *__restrict__ js, double *acc, int npad, float eps2) {
-  __shared__ double lds[kBatchWaves][3][kBatchMaxTargets];
-  __shared__ float4 tiles_lds[kBatchWaves][64];
-  __shared__ int cur;
+template <int W, int IPL>
+__device__ __forceinline__ void batch_item_tile(const BatchItem &it, const float4 *js,
+                                                double *acc, int npad, float eps2,
+                                                double (*lds)[3][kBatchMaxTargets],
+                                                float4 (*tiles_lds)[kBatchMaxTile]) {
+  if (it.tile == 256) batch_item<W, IPL, 256>(it, js, acc, npad, eps2, lds, tiles_lds);
+  else if (it.tile == 128) batch_item<W, IPL, 128>(it, js, acc, npad, eps2, lds, tiles_lds);
+  else batch_item<W, IPL, 64>(it, js, acc, npad, eps2, lds, tiles_lds);
+}
+
+// Persistent grid: each workgroup (W waves) takes items from the queue until it is exhausted, so
+// the grid drains on its own.  W = 1: the four waves of a 256-thread workgroup are independent
+// and each takes its own items.
+template <int W>
+__global__ __launch_bounds__(256) void k_batch_force(const BatchItem *__restrict__ items,
+                                                     int nitems, BatchQueue *queue,
+                                                     const float4 *__restrict__ js, double *acc,
+                                                     int npad, float eps2) {
+  __shared__ double lds[W > 1 ? W : 1][3][kBatchMaxTargets];
+  __shared__ float4 tiles_lds[W > 1 ? W : 4][kBatchMaxTile];
+  __shared__ int cur[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) cur = (int)atomicAdd(&queue->next, 1u);
-    __syncthreads();
-    const int q = cur;
+    int q;
+    if (W == 1) {  // per wave: lane 0 takes the next item, the wave reads it back
+      if (lane == 0) cur[w] = (int)atomicAdd(&queue->next, 1u);
+      wave_lds_sync();
+      q = cur[w];
+      wave_lds_sync();
+    } else {
+      __syncthreads();
+      if (threadIdx.x == 0) cur[0] = (int)atomicAdd(&queue->next, 1u);
+      __syncthreads();
+      q = cur[0];
+    }
     if (q >= nitems) break;
     const BatchItem it = items[q];
     if (it.ipl == 4)
-      batch_item<4>(it, js, acc, npad, eps2, lds, tiles_lds);
+      batch_item_tile<W, 4>(it, js, acc, npad, eps2, lds, tiles_lds);
     else if (it.ipl == 2)
-      batch_item<2>(it, js, acc, npad, eps2, lds, tiles_lds);
+      batch_item_tile<W, 2>(it, js, acc, npad, eps2, lds, tiles_lds);
     else
-      batch_item<1>(it, js, acc, npad, eps2, lds, tiles_lds);
+      batch_item_tile<W, 1>(it, js, acc, npad, eps2, lds, tiles_lds);
   }
 }
 
@@ -194,8 +240,10 @@ struct nb_batch {
   BatchItem *items = nullptr;
   BatchQueue *queue = nullptr;
   int nitems = 0;
-  int grid = 0;
   int bucket_ipl[kBuckets] = {2, 4, 4};
+  int bucket_tile[kBuckets] = {64, 64, 64};
+  int waves = 4;  // waves per work item (1, 2 or 4)
+  int cus = 0;
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -248,8 +296,8 @@ void free_batch(nb_batch *b) {
   b->queue = nullptr;
 }
 
-// Build the item list (longest first) with the per-bucket IPL.
-int build_items(nb_batch *b, int only_bucket, int force_ipl) {
+// Build the item list (longest first) with the per-bucket IPL and tile.
+int build_items(nb_batch *b, int only_bucket, int force_ipl, int force_tile = 0) {
   std::vector<BatchItem> v;
   for (int k = 0; k < b->nsims; k++) {
     const int64_t nk = b->off[k + 1] - b->off[k];
@@ -257,6 +305,7 @@ int build_items(nb_batch *b, int only_bucket, int force_ipl) {
     const int bk = bucket_of(nk);
     if (only_bucket >= 0 && bk != only_bucket) continue;
     const int ipl = force_ipl ? force_ipl : b->bucket_ipl[bk];
+    const int tile = force_tile ? force_tile : b->bucket_tile[bk];
     const int64_t sn = b->poff[k + 1] - b->poff[k];
     const int64_t blk = 64 * ipl;
     for (int64_t t = 0; t < nk; t += blk) {
@@ -266,6 +315,7 @@ int build_items(nb_batch *b, int only_bucket, int force_ipl) {
       it.s0 = (int32_t)b->poff[k];
       it.sn = (int32_t)sn;
       it.ipl = ipl;
+      it.tile = tile;
       v.push_back(it);
     }
   }
@@ -283,9 +333,17 @@ int build_items(nb_batch *b, int only_bucket, int force_ipl) {
 
 hipError_t launch_batch_force(nb_batch *b) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->queue);
-  hipLaunchKernelGGL(k_batch_force, dim3((unsigned)b->grid), dim3(64 * kBatchWaves), 0,
-                     b->stream, b->items, b->nitems, b->queue, b->js, b->acc, (int)b->n,
-                     (float)(b->p.eps * b->p.eps));
+  const float eps2 = (float)(b->p.eps * b->p.eps);
+  // 32 waves per CU: 8 workgroups of 4 waves (W = 4, and W = 1 with independent waves), 16 of 2
+  if (b->waves == 2)
+    hipLaunchKernelGGL(k_batch_force<2>, dim3((unsigned)(b->cus * 16)), dim3(128), 0, b->stream,
+                       b->items, b->nitems, b->queue, b->js, b->acc, (int)b->n, eps2);
+  else if (b->waves == 1)
+    hipLaunchKernelGGL(k_batch_force<1>, dim3((unsigned)(b->cus * 8)), dim3(256), 0, b->stream,
+                       b->items, b->nitems, b->queue, b->js, b->acc, (int)b->n, eps2);
+  else
+    hipLaunchKernelGGL(k_batch_force<4>, dim3((unsigned)(b->cus * 8)), dim3(256), 0, b->stream,
+                       b->items, b->nitems, b->queue, b->js, b->acc, (int)b->n, eps2);
   return hipGetLastError();
 }
 
@@ -340,7 +398,7 @@ int nb_batch_create(const nb_params *p, nb_batch **out) {
     delete b;
     return NB_E_HIP;
   }
-  b->grid = prop.multiProcessorCount * 8;  // 8 workgroups (32 waves) per CU
+  b->cus = prop.multiProcessorCount;
   *out = b;
   return NB_OK;
 }
@@ -481,17 +539,19 @@ int nb_batch_forces(nb_batch *b, double *fx, double *fy, double *fz) {
   if (!b->items) return bfail(b, NB_E_STATE, "nb_batch_forces: nothing loaded");
   NB_BHIP(b, hipSetDevice(b->device));
   NB_BHIP(b, launch_batch_force(b));
-  double *tmp = nullptr;
   const int64_t n = b->n;
-  NB_BHIP(b, hipMalloc((void **)&tmp, (size_t)3 * n * sizeof(double)));
-  hipError_t e = launch_combine_force(b->acc, 1, (int)n, (int)n, b->m, b->p.G, 0, tmp, b->stream);
+  // the per-body accelerations and masses come back to the host, where the force is formed
+  // exactly as the combine kernel forms it (m * (G * a))
+  std::vector<double> a((size_t)3 * n), mh((size_t)n);
+  NB_BHIP(b, hipMemcpyAsync(a.data(), b->acc, a.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            b->stream));
+  NB_BHIP(b, hipMemcpyAsync(mh.data(), b->m, mh.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            b->stream));
+  NB_BHIP(b, hipStreamSynchronize(b->stream));
   std::vector<double> h((size_t)3 * n);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h.data(), tmp, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                       b->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-  (void)hipFree(tmp);
-  NB_BHIP(b, e);
+  for (int64_t i = 0; i < n; i++)
+    for (int c = 0; c < 3; c++)
+      h[(size_t)c * n + i] = mh[(size_t)i] * (b->p.G * a[(size_t)c * n + i]);
   double *o[3] = {fx, fy, fz};
   for (int k = 0; k < b->nsims; k++)
     for (int64_t q = 0; q < b->off[k + 1] - b->off[k]; q++)
@@ -541,6 +601,36 @@ int nb_batch_set_shapes(nb_batch *b, const int32_t *ipl3) {
   return b->items ? build_items(b, -1, 0) : NB_OK;
 }
 
+int nb_batch_get_tuning(nb_batch *b, int32_t *cfg7) {
+  if (!b || !cfg7) return NB_E_INVALID;
+  cfg7[0] = b->waves;
+  for (int k = 0; k < kBuckets; k++) {
+    cfg7[1 + 2 * k] = b->bucket_ipl[k];
+    cfg7[2 + 2 * k] = b->bucket_tile[k];
+  }
+  return NB_OK;
+}
+
+int nb_batch_set_tuning(nb_batch *b, const int32_t *cfg7) {
+  if (!b || !cfg7) return NB_E_INVALID;
+  if (cfg7[0] != 1 && cfg7[0] != 2 && cfg7[0] != 4)
+    return bfail(b, NB_E_INVALID, "nb_batch_set_tuning: waves must be 1, 2 or 4");
+  for (int k = 0; k < kBuckets; k++) {
+    const int ipl = cfg7[1 + 2 * k], tile = cfg7[2 + 2 * k];
+    if ((ipl != 1 && ipl != 2 && ipl != 4) || (tile != 64 && tile != 128 && tile != 256))
+      return bfail(b, NB_E_INVALID, "nb_batch_set_tuning: IPL in {1, 2, 4}, tile in {64, 128, 256}");
+  }
+  b->waves = cfg7[0];
+  for (int k = 0; k < kBuckets; k++) {
+    b->bucket_ipl[k] = cfg7[1 + 2 * k];
+    b->bucket_tile[k] = cfg7[2 + 2 * k];
+  }
+  drop_graph(b);
+  return b->items ? build_items(b, -1, 0) : NB_OK;
+}
+
+// For each waves-per-item choice, every bucket's items alone under each (IPL, tile); the waves
+// choice whose best per-bucket times sum lowest is kept, with those per-bucket shapes.
 int nb_batch_autotune(nb_batch *b, int32_t reps, int32_t *ipl3) {
   if (!b) return NB_E_INVALID;
   if (!b->items) return bfail(b, NB_E_STATE, "nb_batch_autotune: nothing loaded");
@@ -549,29 +639,47 @@ int nb_batch_autotune(nb_batch *b, int32_t reps, int32_t *ipl3) {
   hipEvent_t e0, e1;
   NB_BHIP(b, hipEventCreate(&e0));
   NB_BHIP(b, hipEventCreate(&e1));
-  for (int bk = 0; bk < kBuckets; bk++) {
-    float best = 1e30f;
-    int best_ipl = b->bucket_ipl[bk];
-    for (int ipl : {1, 2, 4}) {
-      int rc = build_items(b, bk, ipl);
-      if (rc) return rc;
-      if (b->nitems == 0) break;
-      NB_BHIP(b, launch_batch_force(b));  // warm-up
-      NB_BHIP(b, hipEventRecord(e0, b->stream));
-      for (int r = 0; r < reps; r++) NB_BHIP(b, launch_batch_force(b));
-      NB_BHIP(b, hipEventRecord(e1, b->stream));
-      NB_BHIP(b, hipEventSynchronize(e1));
-      float ms = 0;
-      NB_BHIP(b, hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        best_ipl = ipl;
-      }
+  float best_total = 1e30f;
+  int best_w = b->waves, best_ipl[kBuckets], best_tile[kBuckets];
+  for (int k = 0; k < kBuckets; k++) best_ipl[k] = b->bucket_ipl[k], best_tile[k] = b->bucket_tile[k];
+  for (int w : {1, 2, 4}) {
+    b->waves = w;
+    float total = 0;
+    int ipl_w[kBuckets], tile_w[kBuckets];
+    for (int bk = 0; bk < kBuckets; bk++) {
+      float best = 1e30f;
+      ipl_w[bk] = b->bucket_ipl[bk];
+      tile_w[bk] = b->bucket_tile[bk];
+      for (int ipl : {1, 2, 4})
+        for (int tile : {64, 128, 256}) {
+          int rc = build_items(b, bk, ipl, tile);
+          if (rc) return rc;
+          if (b->nitems == 0) break;
+          NB_BHIP(b, launch_batch_force(b));  // warm-up
+          NB_BHIP(b, hipEventRecord(e0, b->stream));
+          for (int r = 0; r < reps; r++) NB_BHIP(b, launch_batch_force(b));
+          NB_BHIP(b, hipEventRecord(e1, b->stream));
+          NB_BHIP(b, hipEventSynchronize(e1));
+          float ms = 0;
+          NB_BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            ipl_w[bk] = ipl;
+            tile_w[bk] = tile;
+          }
+        }
+      if (best < 1e30f) total += best;
     }
-    b->bucket_ipl[bk] = best_ipl;
+    if (total < best_total) {
+      best_total = total;
+      best_w = w;
+      for (int k = 0; k < kBuckets; k++) best_ipl[k] = ipl_w[k], best_tile[k] = tile_w[k];
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  b->waves = best_w;
+  for (int k = 0; k < kBuckets; k++) b->bucket_ipl[k] = best_ipl[k], b->bucket_tile[k] = best_tile[k];
   int rc = build_items(b, -1, 0);
   if (rc) return rc;
   if (ipl3)

@@ -486,6 +486,7 @@ struct BhForceArgs {
   int npad;
   int gsz;  // bodies per wave (lanes gsz..63 idle)
   double G, theta, soft2;
+  int64_t node_cap;
 };
 
 // One interaction, op for op as accumForce / directForce (octree.go:292-298, 314-325).
@@ -532,11 +533,19 @@ __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
     // One code path for leaves and nodes, so that lanes at different kinds of slot share the
     // square root and the term: a leaf (the body's centre of mass (x*m)/m and mass 0 + m) is
     // taken unless it is the body itself, a node when its opening test passes.
-    // The slot's 64 bytes are read as four 16-byte words at the top of the iteration, so the
-    // walk waits for one memory round trip per visited slot.
+    // The walk is pipelined by one slot: once the test has chosen the next slot, its 64 bytes
+    // are requested before the current term is computed, so the term's divisions run while the
+    // load is in flight.  The arithmetic and its order are unchanged.
+    int4 w0, w1, w2, w3;
+    auto fetch = [&](int slot) {
+      const int4 *w = reinterpret_cast<const int4 *>(a.slots + slot);
+      w0 = w[0];
+      w1 = w[1];
+      w2 = w[2];
+      w3 = w[3];
+    };
+    if (q >= 0) fetch(q);
     while (q >= 0) {
-      const int4 *w = reinterpret_cast<const int4 *>(a.slots + q);
-      const int4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
       BhSlot sl;
       __builtin_memcpy(&sl, &w0, 16);
       __builtin_memcpy(reinterpret_cast<char *>(&sl) + 16, &w1, 16);
@@ -555,22 +564,277 @@ __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
           nint++;
         }
         q = sl.skip;
+        if (q >= 0) fetch(q);
         continue;
       }
       const double distSq = dx * dx + dy * dy + dz * dz + soft2;
       const double dist = __builtin_sqrt(distSq);
       const bool take = leaf ? (-2 - sl.code != s) : ((2 * sl.half / dist) < theta);
+      q = (leaf || take) ? sl.skip : sl.first;
+      if (q >= 0) fetch(q);
       if (take) {
         bh_term(dx, dy, dz, distSq, dist, Gm, sl.m, fx, fy, fz);
         nint++;
       }
-      q = (leaf || take) ? sl.skip : sl.first;
     }
     a.acc[i] = fx;
     a.acc[a.npad + i] = fy;
     a.acc[2 * a.npad + i] = fz;
   }
   if (nact == 0) return;
+  for (int o = 32; o > 0; o >>= 1) nint += __shfl_xor(nint, o);
+  if (lane == 0) atomicAdd(&a.cnt_w->interactions, nint);
+}
+
+// ---- fast mode: one walk per group of bodies, FP32 terms ------------------------------------------
+// The 64 bodies of a wave (consecutive in key order, so close in space) share one traversal.  A
+// wave whose bodies straddle separate regions of key order (its box more than 8 times the summed
+// volume of its four 16-body quarters' boxes) walks the tree once per quarter instead.  A
+// node is accepted for the whole group when 2 half < theta * sqrt(d_box^2 + soft^2), d_box being
+// its centre of mass's distance to the group's bounding box: every body of the group is then at
+// least that far away, so each would accept it by the reference's own test (octree.go:299-309);
+// otherwise the node is opened.  The walk is breadth-parallel: each step pops up to 8 nodes from
+// a per-wave LDS stack and the 64 lanes examine their 64 child slots at once; accepted nodes and
+// leaves go to an LDS list of terms, opened nodes back to the stack.  Every 64 listed terms are
+// evaluated by all lanes against their own bodies like the direct FP32 kernel (sources broadcast
+// from LDS, coordinates relative to the group's centre, r^-3 from v_rsq_f32), the list's partial
+// sums are added into FP64 per lane, and the self term is skipped by body index.
+constexpr int kBhFastStack = 2048;  // per-wave stack entries (bound: 64 per level, 21 levels)
+constexpr int kBhFastList = 128;    // per-wave term list (< 64 kept + up to 64 added per step)
+
+struct BhFastLds {
+  int32_t stack[kBhFastStack];
+  float4 term[kBhFastList];
+  int32_t tidx[kBhFastList];  // body (sorted index) of a leaf term, -1 for a node
+  int32_t top, len;
+};
+
+__device__ __forceinline__ void fast_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The first `cnt` listed terms against the lane's body; the rest of the list moves to the front.
+__device__ __forceinline__ void fast_flush(BhFastLds &L, int cnt, float tx, float ty, float tz,
+                                           float soft2, int s, bool act, double &ax, double &ay,
+                                           double &az, unsigned long long &nint) {
+  const int lane = threadIdx.x & 63;
+  float px = 0.f, py = 0.f, pz = 0.f;
+  const float sf = to_vgpr(soft2);
+  for (int j = 0; j < cnt; j++) {
+    const float4 e = L.term[j];
+    const int id = L.tidx[j];
+    const float dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+    float r2 = __builtin_fmaf(dx, dx, sf);
+    r2 = __builtin_fmaf(dy, dy, r2);
+    r2 = __builtin_fmaf(dz, dz, r2);
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    float w = e.w * inv;
+    w = w * (inv * inv);
+    w = id == s ? 0.f : w;
+    px = __builtin_fmaf(dx, w, px);
+    py = __builtin_fmaf(dy, w, py);
+    pz = __builtin_fmaf(dz, w, pz);
+  }
+  if (act) {
+    ax += px;
+    ay += py;
+    az += pz;
+    nint += (unsigned long long)cnt;
+  }
+  fast_sync();
+  const int rest = L.len - cnt;
+  float4 mv = make_float4(0.f, 0.f, 0.f, 0.f);
+  int mi = -1;
+  if (lane < rest) {
+    mv = L.term[cnt + lane];
+    mi = L.tidx[cnt + lane];
+  }
+  fast_sync();
+  if (lane < rest) {
+    L.term[lane] = mv;
+    L.tidx[lane] = mi;
+  }
+  if (lane == 0) L.len = rest;
+  fast_sync();
+}
+
+// Appends one term per lane that has one (wave-wide, in lane order).
+__device__ __forceinline__ void fast_append(BhFastLds &L, bool has, float4 t, int id) {
+  const unsigned long long bal = __builtin_amdgcn_ballot_w64(has);
+  const int base = L.len;
+  if (has) {
+    const int pos = base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+    L.term[pos] = t;
+    L.tidx[pos] = id;
+  }
+  fast_sync();
+  if ((threadIdx.x & 63) == 0) L.len = base + __builtin_popcountll(bal);
+  fast_sync();
+}
+
+__global__ __launch_bounds__(256) void k_bh_force_fast(BhForceArgs a) {
+  __shared__ BhFastLds lds[4];
+  const int M = a.cnt->M;
+  if (M < 2) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  BhFastLds &L = lds[wv];
+  const int s = (blockIdx.x * 4 + wv) * 64 + lane;
+  bool act = s < M;
+  int i = -1;
+  if (act) {
+    i = a.perm[s];
+    act = !(a.flags[i] & NB_F_SUN);  // suns sum directly
+  }
+  if (__builtin_amdgcn_ballot_w64(act) == 0) return;
+  const double px = act ? a.sx[s] : 0, py = act ? a.sy[s] : 0, pz = act ? a.sz[s] : 0;
+  // boxes of the 16-lane quarters and of the whole wave (FP64)
+  double lo[3] = {act ? px : DBL_MAX, act ? py : DBL_MAX, act ? pz : DBL_MAX};
+  double hi[3] = {act ? px : -DBL_MAX, act ? py : -DBL_MAX, act ? pz : -DBL_MAX};
+  for (int o = 8; o > 0; o >>= 1)
+    for (int d = 0; d < 3; d++) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+    }
+  double qlo[3] = {lo[0], lo[1], lo[2]}, qhi[3] = {hi[0], hi[1], hi[2]};  // this quarter's box
+  auto vol = [](const double *l, const double *h) {
+    return l[0] <= h[0] ? (h[0] - l[0]) * (h[1] - l[1]) * (h[2] - l[2]) : 0.0;
+  };
+  double qv = vol(qlo, qhi);
+  for (int o = 32; o >= 16; o >>= 1) {
+    for (int d = 0; d < 3; d++) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+    }
+  }
+  double sumq = qv;  // sum of the four quarter volumes (each quarter's lanes hold its own)
+  sumq += __shfl_xor(sumq, 16);
+  sumq += __shfl_xor(sumq, 32);
+  // a wave whose bodies straddle separate regions (its box far larger than its quarters' boxes)
+  // walks the tree once per quarter, with the quarter's box; otherwise once for all 64 bodies
+  const bool split = vol(lo, hi) > 8.0 * sumq + 1e-300;
+  const int ngroups = split ? 4 : 1;
+  const double th2 = a.theta * a.theta, soft2 = a.soft2;
+  double ax = 0, ay = 0, az = 0;
+  unsigned long long nint = 0;
+  bool overflow = false;
+  const int node_lim = (int)min(a.node_cap, (int64_t)INT32_MAX / 8);
+  for (int grp = 0; grp < ngroups && !overflow; grp++) {
+    double glo[3], ghi[3];
+    for (int d = 0; d < 3; d++) {
+      glo[d] = split ? __shfl(qlo[d], grp * 16) : lo[d];
+      ghi[d] = split ? __shfl(qhi[d], grp * 16) : hi[d];
+    }
+    const bool mine = act && (!split || (lane >> 4) == grp);
+    if (__builtin_amdgcn_ballot_w64(mine) == 0) continue;
+    const double cx = 0.5 * (glo[0] + ghi[0]), cy = 0.5 * (glo[1] + ghi[1]),
+                 cz = 0.5 * (glo[2] + ghi[2]);
+    const double hx = 0.5 * (ghi[0] - glo[0]), hy = 0.5 * (ghi[1] - glo[1]),
+                 hz = 0.5 * (ghi[2] - glo[2]);
+    const float tx = (float)(px - cx), ty = (float)(py - cy), tz = (float)(pz - cz);
+    // a node accepted for the group: (2 half)^2 < theta^2 (d_box^2 + soft^2)
+    auto accept = [&](double x, double y, double z, double half) {
+      const double ex = fmax(fabs(x - cx) - hx, 0.0), ey = fmax(fabs(y - cy) - hy, 0.0),
+                   ez = fmax(fabs(z - cz) - hz, 0.0);
+      return 4.0 * half * half < th2 * (ex * ex + ey * ey + ez * ez + soft2);
+    };
+    if (lane == 0) {
+      L.top = 0;
+      L.len = 0;
+    }
+    fast_sync();
+    const double4 root = a.nmc[0];
+    if (root.w != 0.0) {
+      if (accept(root.x, root.y, root.z, a.nbox[0].w)) {
+        fast_append(L, lane == 0, make_float4((float)(root.x - cx), (float)(root.y - cy),
+                                              (float)(root.z - cz), (float)root.w), -1);
+      } else {
+        if (lane == 0) {
+          L.stack[0] = 0;
+          L.top = 1;
+        }
+        fast_sync();
+      }
+    }
+    for (;;) {
+      const int top = L.top;
+      if (top == 0) break;
+      const int nn = min(8, top);
+      const int k = lane >> 3, c = lane & 7;
+      const int node = k < nn ? L.stack[top - nn + k] : -1;
+      fast_sync();
+      if (lane == 0) L.top = top - nn;
+      fast_sync();
+      bool term = false, open = false, bucket = false;
+      float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+      int id = -1, code = -1;
+      if (node >= 0 && node < node_lim) {
+        const BhSlot sl = a.slots[(size_t)node * 8 + c];
+        code = sl.code;
+        if (code != -1 && sl.m > 0) {
+          if (code <= -2) {  // a leaf: the body's own centre of mass (x*m)/m and mass 0 + m
+            term = true;
+            id = -2 - code;
+          } else if (sl.half < 0) {
+            bucket = code < node_lim;
+          } else if (accept(sl.x, sl.y, sl.z, sl.half)) {
+            term = true;
+          } else {
+            open = code < node_lim;
+          }
+          if (term)
+            t = make_float4((float)(sl.x - cx), (float)(sl.y - cy), (float)(sl.z - cz),
+                            (float)sl.m);
+        }
+      }
+      fast_append(L, term, t, id);
+      if (L.len >= 64) fast_flush(L, 64, tx, ty, tz, (float)soft2, s, mine, ax, ay, az, nint);
+      // opened nodes onto the stack
+      const unsigned long long bo = __builtin_amdgcn_ballot_w64(open);
+      const int base = L.top, nopen = __builtin_popcountll(bo);
+      if (base + nopen > kBhFastStack) {
+        overflow = true;
+        break;
+      }
+      if (open)
+        L.stack[base + (int)__builtin_amdgcn_mbcnt_hi(
+                           (uint32_t)(bo >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bo, 0))] =
+            code;
+      fast_sync();
+      if (lane == 0) L.top = base + nopen;
+      fast_sync();
+      // buckets (the depth limit): their bodies one by one, 64 at a time
+      unsigned long long bb = __builtin_amdgcn_ballot_w64(bucket);
+      while (bb) {
+        const int src = __builtin_ctzll(bb);
+        bb &= bb - 1;
+        const int bnode = __shfl(code, src);
+        const int32_t *ch = a.nchild + (size_t)bnode * 8;
+        const int b0 = ch[1], bn = ch[2];
+        for (int q0 = 0; q0 < bn; q0 += 64) {
+          const int q = b0 + q0 + lane;
+          const bool h = q0 + lane < bn;
+          float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (h)
+            e = make_float4((float)(a.sx[q] - cx), (float)(a.sy[q] - cy), (float)(a.sz[q] - cz),
+                            (float)a.sm[q]);
+          fast_append(L, h, e, q);
+          if (L.len >= 64)
+            fast_flush(L, 64, tx, ty, tz, (float)soft2, s, mine, ax, ay, az, nint);
+        }
+      }
+    }
+    if (L.len > 0) fast_flush(L, L.len, tx, ty, tz, (float)soft2, s, mine, ax, ay, az, nint);
+  }
+  if (overflow && lane == 0) atomicOr(&a.cnt_w->stack_overflow, 1u);
+  if (act) {
+    const double Gm = a.G * a.sm[s];
+    a.acc[i] = Gm * ax;
+    a.acc[a.npad + i] = Gm * ay;
+    a.acc[2 * a.npad + i] = Gm * az;
+  }
   for (int o = 32; o > 0; o >>= 1) nint += __shfl_xor(nint, o);
   if (lane == 0) atomicAdd(&a.cnt_w->interactions, nint);
 }
@@ -882,14 +1146,18 @@ int bh_force_phase(Ctx *c, bool timed, int ev_slot) {
   // the shared part of the walk in more waves (measured 6.9 / 8.4 / 10.7 ms at N = 65,536 for
   // 64 / 32 / 16)
   fa.gsz = b->p.group ? b->p.group : 64;
+  fa.node_cap = b->node_cap;
   fa.acc = c->acc;
   fa.npad = (int)c->npad;
   fa.G = c->p.G;
   fa.theta = b->p.theta;
   fa.soft2 = b->p.softening * b->p.softening;
   if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot], st));
-  hipLaunchKernelGGL(k_bh_force, dim3((unsigned)((n + 4 * fa.gsz - 1) / (4 * fa.gsz))), dim3(256),
-                     0, st, fa);
+  if (b->p.mode == 1)
+    hipLaunchKernelGGL(k_bh_force_fast, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, fa);
+  else
+    hipLaunchKernelGGL(k_bh_force, dim3((unsigned)((n + 4 * fa.gsz - 1) / (4 * fa.gsz))),
+                       dim3(256), 0, st, fa);
   hipLaunchKernelGGL(k_bh_sun_list, dim3(nb256(n)), dim3(256), 0, st, c->flags, n, b->cnt,
                      b->suns);
   BhSunArgs sa;
@@ -920,7 +1188,10 @@ int bh_force_phase(Ctx *c, bool timed, int ev_slot) {
     da.events = c->p.events;
     hipLaunchKernelGGL(k_bh_detect, dim3(nb256(n)), dim3(256), 0, st, da);
     NB_HIP(c, hipGetLastError());
-    NB_HIP(c, launch_sort_pairs(c->pairs, c->pair_cap, c->ctr, st));
+    {
+      const int rcs = sort_pairs(c);
+      if (rcs) return rcs;
+    }
   }
   b->builds++;
   return NB_OK;
@@ -950,9 +1221,11 @@ int nb_bh_enable(nb_ctx *c, const nb_bh_params *p) {
     nb_bh_params_init(&q);
   if (!(q.theta >= 0) || !std::isfinite(q.theta) || !(q.softening >= 0) ||
       !std::isfinite(q.softening) || q.max_depth < 1 || q.max_depth > kBhLevels ||
-      !(q.group == 0 || q.group == 16 || q.group == 32 || q.group == 64))
+      !(q.group == 0 || q.group == 16 || q.group == 32 || q.group == 64) ||
+      (q.mode != 0 && q.mode != 1))
     return fail_msg(c, NB_E_INVALID,
-                    "nb_bh_enable: theta, softening >= 0, max_depth in [1, 21], group 0/16/32/64");
+                    "nb_bh_enable: theta, softening >= 0, max_depth in [1, 21], group 0/16/32/64, "
+                    "mode 0/1");
   if (c->p.precision != NB_PREC_FP64)
     return fail_msg(c, NB_E_UNSUPPORTED, "nb_bh_enable: the octree path is FP64 (precision fp64)");
   if (c->sur_on)

@@ -7,6 +7,8 @@
 #include <cfloat>
 #include <cmath>
 
+#include <algorithm>
+
 #include "nb_internal.hpp"
 
 namespace nb {
@@ -148,65 +150,104 @@ hipError_t launch_accel_f64(const Accel64Args &a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Pair sort: keys (i << 32 | j) ascending = canonical order.  One workgroup; bitonic network in
-// LDS up to kSortLdsKeys keys, in global memory (padded to a power of two) beyond.
+// Pair sort: keys (i << 32 | j) ascending = canonical order.  The count is known on the device
+// only, so the host launches a fixed sequence sized by the capacity and each kernel reads the
+// count: kSortLdsKeys-key chunks are sorted in LDS by one workgroup each (one workgroup sorts the
+// whole list when it fits), then runs of doubling length are merged (each key finds its place by
+// a binary search in the other run), one launch per run length up to the capacity.  A pass whose
+// runs already span the list copies it, so the buffers alternate in a fixed order and the result
+// always ends in the key buffer.
 // ---------------------------------------------------------------------------------------------
-// keys has room for the next power of two above cap.
-__global__ __launch_bounds__(1024) void k_sort_pairs(uint64_t *keys, int64_t cap,
-                                                     const DevCounters *ctr) {
-  __shared__ uint64_t sh[kSortLdsKeys];
-  const int64_t cnt = min((int64_t)ctr->pair_count, cap);
-  if (cnt <= 1) return;
-  int64_t P = 1;
-  while (P < cnt) P <<= 1;
+// Sorts sh[0, P) ascending (P a power of two) with the block's threads.
+__device__ static void bitonic_lds(uint64_t *sh, int64_t P) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  if (P <= kSortLdsKeys) {
-    for (int64_t q = tid; q < P; q += nt) sh[q] = q < cnt ? keys[q] : ~0ull;
-    __syncthreads();
-    for (int64_t k = 2; k <= P; k <<= 1) {
-      for (int64_t j = k >> 1; j > 0; j >>= 1) {
-        for (int64_t q = tid; q < P; q += nt) {
-          const int64_t o = q ^ j;
-          if (o > q) {
-            const uint64_t u = sh[q], v = sh[o];
-            const bool up = (q & k) == 0;
-            if ((u > v) == up) {
-              sh[q] = v;
-              sh[o] = u;
-            }
+  for (int64_t k = 2; k <= P; k <<= 1) {
+    for (int64_t j = k >> 1; j > 0; j >>= 1) {
+      for (int64_t q = tid; q < P; q += nt) {
+        const int64_t o = q ^ j;
+        if (o > q) {
+          const uint64_t u = sh[q], v = sh[o];
+          const bool up = (q & k) == 0;
+          if ((u > v) == up) {
+            sh[q] = v;
+            sh[o] = u;
           }
         }
-        __syncthreads();
       }
-    }
-    for (int64_t q = tid; q < cnt; q += nt) keys[q] = sh[q];
-  } else {
-    for (int64_t q = cnt + tid; q < P; q += nt) keys[q] = ~0ull;
-    __threadfence_block();
-    __syncthreads();
-    for (int64_t k = 2; k <= P; k <<= 1) {
-      for (int64_t j = k >> 1; j > 0; j >>= 1) {
-        for (int64_t q = tid; q < P; q += nt) {
-          const int64_t o = q ^ j;
-          if (o > q) {
-            const uint64_t u = keys[q], v = keys[o];
-            const bool up = (q & k) == 0;
-            if ((u > v) == up) {
-              keys[q] = v;
-              keys[o] = u;
-            }
-          }
-        }
-        __threadfence_block();
-        __syncthreads();
-      }
+      __syncthreads();
     }
   }
 }
 
-hipError_t launch_sort_pairs(uint64_t *keys, int64_t cap, const DevCounters *ctr,
+// src[base, base + cnt) (cnt <= kSortLdsKeys) sorted through LDS into dst[base, base + cnt).
+__device__ static void sort_chunk_lds(uint64_t *sh, const uint64_t *src, uint64_t *dst,
+                                      int64_t base, int64_t cnt) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  int64_t P = 1;
+  while (P < cnt) P <<= 1;
+  for (int64_t q = tid; q < P; q += nt) sh[q] = q < cnt ? src[base + q] : ~0ull;
+  __syncthreads();
+  bitonic_lds(sh, P);
+  for (int64_t q = tid; q < cnt; q += nt) dst[base + q] = sh[q];
+}
+
+__global__ __launch_bounds__(1024) void k_sort_chunks(const uint64_t *keys, uint64_t *dst,
+                                                      int64_t cap, const DevCounters *ctr) {
+  __shared__ uint64_t sh[kSortLdsKeys];
+  const int64_t cnt = min((int64_t)ctr->pair_count, cap);
+  const int64_t base = (int64_t)blockIdx.x * kSortLdsKeys;
+  if (base >= cnt) return;
+  sort_chunk_lds(sh, keys, dst, base, min((int64_t)kSortLdsKeys, cnt - base));
+}
+
+// Number of entries of the sorted run r[0, n) below key (strict) or, with upper, not above it.
+__device__ __forceinline__ int64_t run_rank(const uint64_t *r, int64_t n, uint64_t key,
+                                            bool upper) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const uint64_t v = r[mid];
+    if (v < key || (upper && v == key)) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Runs of length L in src merged pairwise into dst (copied when one run spans the list).
+__global__ __launch_bounds__(256) void k_merge_pass(const uint64_t *src, uint64_t *dst,
+                                                    int64_t L, int64_t cap,
+                                                    const DevCounters *ctr) {
+  const int64_t cnt = min((int64_t)ctr->pair_count, cap);
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gs) {
+    const uint64_t key = src[q];
+    if (cnt <= L) {
+      dst[q] = key;
+      continue;
+    }
+    const int64_t r = q / L, m0 = (r & ~(int64_t)1) * L;
+    if (!(r & 1)) {  // left run: place after the right run's smaller keys
+      const int64_t rb = m0 + L;
+      dst[rb < cnt ? q + run_rank(src + rb, min(L, cnt - rb), key, false) : q] = key;
+    } else {  // right run: after the left run's keys not above it (stable)
+      dst[q - L + run_rank(src + m0, L, key, true)] = key;
+    }
+  }
+}
+
+hipError_t launch_sort_pairs(uint64_t *keys, uint64_t *tmp, int64_t cap, const DevCounters *ctr,
                              hipStream_t st) {
-  hipLaunchKernelGGL(k_sort_pairs, dim3(1), dim3(1024), 0, st, keys, cap, ctr);
+  int passes = 0;
+  for (int64_t L = kSortLdsKeys; L < cap; L <<= 1) passes++;
+  // the chunks go to whichever buffer makes the last pass land in keys
+  uint64_t *buf[2] = {(passes & 1) ? tmp : keys, (passes & 1) ? keys : tmp};
+  const unsigned chunks = (unsigned)((cap + kSortLdsKeys - 1) / kSortLdsKeys);
+  hipLaunchKernelGGL(k_sort_chunks, dim3(chunks), dim3(1024), 0, st, keys, buf[0], cap, ctr);
+  const unsigned g = (unsigned)std::min<int64_t>(2048, (cap + 255) / 256);
+  int64_t L = kSortLdsKeys;
+  for (int p = 0; p < passes; p++, L <<= 1)
+    hipLaunchKernelGGL(k_merge_pass, dim3(g), dim3(256), 0, st, buf[p & 1], buf[(p + 1) & 1], L,
+                       cap, ctr);
   return hipGetLastError();
 }
 
@@ -371,72 +412,223 @@ __device__ static void initiate_fragmentation(const ResolveArgs &a, int b, doubl
   a.fst[b] = st;
 }
 
+// Tallies of one resolution pass.
+struct ResolveTally {
+  uint64_t resolved = 0, fragreq = 0, subsumed = 0, started = 0, inactive = 0;
+};
+
+// Whether the pair (lo, hi) can change anything when events are off: an Elastic first body
+// (body.go:252-253) and both bodies existing (:249-251).  With events off nothing in the pass
+// changes behaviours or flags, so this is decided once per pass.  b receives the first body.
+__device__ static bool elastic_pair(const ResolveArgs &a, int lo, int hi, int *b, int *o) {
+  if (a.beh[lo] == NB_CB_ELASTIC && collide_b(a.beh[hi])) {
+    *b = lo;
+    *o = hi;
+  } else if (a.beh[hi] == NB_CB_ELASTIC && collide_b(a.beh[lo])) {
+    *b = hi;
+    *o = lo;
+  } else {
+    return false;
+  }
+  return (a.flags[*b] & NB_F_EXISTS) && (a.flags[*o] & NB_F_EXISTS);
+}
+
+// One canonical pair, as ProcessMods handles its event (body_collection.go:212-233).
+__device__ static void resolve_pair(const ResolveArgs &a, int lo, int hi, ResolveTally &t) {
+  if (a.events && !(collide_b(a.beh[lo]) && collide_b(a.beh[hi]))) {
+    // a subsume pair: ResolveSubsume (body.go:228-244) by the larger body, no existence check
+    const int b = a.r[lo] > a.r[hi] ? lo : hi, o = b == lo ? hi : lo;
+    a.m[b] = a.m[b] + a.m[o];
+    const uint8_t fo = a.flags[o];
+    if ((fo & NB_F_EXISTS) && !(fo & NB_F_FRAGMENTING)) t.inactive++;
+    a.flags[o] = (uint8_t)(fo & ~NB_F_EXISTS);  // SetNotExists (body.go:93-96)
+    a.m[o] = 0.0;
+    t.subsumed++;
+    return;
+  }
+  int b, o;
+  if (!elastic_pair(a, lo, hi, &b, &o)) return;
+  Elastic e;
+  if (!elastic(a, b, o, e)) return;
+  t.resolved++;
+  double tf = 0, of = 0;
+  if (should_fragment(a, b, o, e, &tf, &of)) {
+    t.fragreq++;
+    if (a.events) {  // doFragment (fragcalc.go:54-63)
+      if (a.beh[b] == NB_CB_FRAGMENT && tf > a.frag[b])
+        initiate_fragmentation(a, b, tf, &t.started, &t.inactive);
+      if (a.beh[o] == NB_CB_FRAGMENT && of > a.frag[o])
+        initiate_fragmentation(a, o, of, &t.started, &t.inactive);
+    }
+    return;
+  }
+  // doElastic (collisioncalc.go:26-35): restitution of the first body.
+  const double rr = a.rest[b];
+  a.vx[b] = (e.v1[0] - e.cm[0]) * rr + e.cm[0];
+  a.vy[b] = (e.v1[1] - e.cm[1]) * rr + e.cm[1];
+  a.vz[b] = (e.v1[2] - e.cm[2]) * rr + e.cm[2];
+  a.vx[o] = (e.v2[0] - e.cm[0]) * rr + e.cm[0];
+  a.vy[o] = (e.v2[1] - e.cm[1]) * rr + e.cm[1];
+  a.vz[o] = (e.v2[2] - e.cm[2]) * rr + e.cm[2];
+  a.collided[b] = 1;
+  a.collided[o] = 1;
+}
+
+__device__ static void add_tally(DevCounters *ctr, uint32_t found, const ResolveTally &t) {
+  ctr->pairs_total += found;
+  ctr->pairs_last = found;
+  ctr->resolved_total += t.resolved;
+  ctr->fragment_requests += t.fragreq;
+  ctr->subsumed_total += t.subsumed;
+  ctr->frag_started += t.started;
+  ctr->inactive_events += t.inactive;
+}
+
+// The pass one pair at a time, in canonical order, on one lane: the ordering reference, and the
+// pass used with events on (subsumes and fragmentation ids depend on the order across pairs).
 __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   if (threadIdx.x != 0) return;
   const uint32_t found = a.ctr->pair_count;
   const int64_t cnt = min((int64_t)found, a.pair_cap);
-  uint64_t resolved = 0, fragreq = 0, subsumed = 0, started = 0, inactive = 0;
+  ResolveTally t;
   for (int64_t k = 0; k < cnt; k++) {
     const uint64_t key = a.pairs[k];
-    const int lo = (int)(key >> 32), hi = (int)(key & 0xffffffffu);
-    if (a.events && !(collide_b(a.beh[lo]) && collide_b(a.beh[hi]))) {
-      // a subsume pair: ResolveSubsume (body.go:228-244) by the larger body, no existence check
-      const int b = a.r[lo] > a.r[hi] ? lo : hi, o = b == lo ? hi : lo;
-      a.m[b] = a.m[b] + a.m[o];
-      const uint8_t fo = a.flags[o];
-      if ((fo & NB_F_EXISTS) && !(fo & NB_F_FRAGMENTING)) inactive++;
-      a.flags[o] = (uint8_t)(fo & ~NB_F_EXISTS);  // SetNotExists (body.go:93-96)
-      a.m[o] = 0.0;
-      subsumed++;
-      continue;
-    }
-    int b, o;
-    if (a.beh[lo] == NB_CB_ELASTIC && collide_b(a.beh[hi])) {
-      b = lo;
-      o = hi;
-    } else if (a.beh[hi] == NB_CB_ELASTIC && collide_b(a.beh[lo])) {
-      b = hi;
-      o = lo;
-    } else {
-      continue;
-    }
-    if (!(a.flags[b] & NB_F_EXISTS) || !(a.flags[o] & NB_F_EXISTS)) continue;
-    Elastic e;
-    if (!elastic(a, b, o, e)) continue;
-    resolved++;
-    double tf = 0, of = 0;
-    if (should_fragment(a, b, o, e, &tf, &of)) {
-      fragreq++;
-      if (a.events) {  // doFragment (fragcalc.go:54-63)
-        if (a.beh[b] == NB_CB_FRAGMENT && tf > a.frag[b])
-          initiate_fragmentation(a, b, tf, &started, &inactive);
-        if (a.beh[o] == NB_CB_FRAGMENT && of > a.frag[o])
-          initiate_fragmentation(a, o, of, &started, &inactive);
-      }
-      continue;
-    }
-    // doElastic (collisioncalc.go:26-35): restitution of the first body.
-    const double rr = a.rest[b];
-    a.vx[b] = (e.v1[0] - e.cm[0]) * rr + e.cm[0];
-    a.vy[b] = (e.v1[1] - e.cm[1]) * rr + e.cm[1];
-    a.vz[b] = (e.v1[2] - e.cm[2]) * rr + e.cm[2];
-    a.vx[o] = (e.v2[0] - e.cm[0]) * rr + e.cm[0];
-    a.vy[o] = (e.v2[1] - e.cm[1]) * rr + e.cm[1];
-    a.vz[o] = (e.v2[2] - e.cm[2]) * rr + e.cm[2];
-    a.collided[b] = 1;
-    a.collided[o] = 1;
+    resolve_pair(a, (int)(key >> 32), (int)(key & 0xffffffffu), t);
   }
-  a.ctr->pairs_total += found;
-  a.ctr->pairs_last = found;
-  a.ctr->resolved_total += resolved;
-  a.ctr->fragment_requests += fragreq;
-  a.ctr->subsumed_total += subsumed;
-  a.ctr->frag_started += started;
-  a.ctr->inactive_events += inactive;
+  add_tally(a.ctr, found, t);
+}
+
+// The same pass with pairs that share no body resolved together (events off).  A pair reads and
+// writes only its two bodies, and nothing in the pass changes which pairs act, so the state is
+// bit-identical to the one-by-one pass as long as every pair sees both its bodies exactly as the
+// pairs before it in canonical order left them.  The pairs touching one body b form a chain in
+// canonical order: the pairs (x, b), x ascending, then b's own pairs (b, y), y ascending.  A pair
+// therefore waits for at most two predecessors (the previous pair on each of its bodies) and
+// releases at most two successors.  The chains come from two sorted lists: the pair keys
+// (lo << 32 | hi) and the same keys swapped (hi << 32 | lo), where b's pairs (x, b) are one run.
+// Then one workgroup resolves the ready pairs round by round: pairs of one round share no body.
+__device__ __forceinline__ int64_t key_lower_bound(const uint64_t *k, int64_t n, uint64_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (k[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// keys_sw[k] = the k-th pair key with its halves swapped (sorted afterwards).
+__global__ void k_pairs_swap(const uint64_t *keys, uint64_t *keys_sw, int64_t cap,
+                             const DevCounters *ctr) {
+  const int64_t cnt = min((int64_t)ctr->pair_count, cap);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = keys[k];
+    keys_sw[k] = (v << 32) | (v >> 32);
+  }
+}
+
+// For each pair: its successor on each of its two bodies (-1: none) and how many predecessors it
+// waits for.  keys: sorted pair keys; sw: sorted swapped keys.
+__global__ void k_pair_dag(const uint64_t *keys, const uint64_t *sw, int64_t cap,
+                           const DevCounters *ctr, int32_t *succ, int32_t *pending) {
+  const int64_t cnt = min((int64_t)ctr->pair_count, cap);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = keys[k];
+    const uint32_t lo = (uint32_t)(key >> 32), hi = (uint32_t)key;
+    int npred = 0;
+    // chain of lo: k lies in lo's own run (lo, y)
+    int32_t s_lo = -1;
+    if (k + 1 < cnt && (uint32_t)(keys[k + 1] >> 32) == lo) s_lo = (int32_t)(k + 1);
+    if (k > 0 && (uint32_t)(keys[k - 1] >> 32) == lo) {
+      npred++;
+    } else {  // the first of lo's own run: after the last pair (x, lo), if there is one
+      const int64_t e = key_lower_bound(sw, cnt, ((uint64_t)lo + 1) << 32) - 1;
+      if (e >= 0 && (uint32_t)(sw[e] >> 32) == lo) npred++;
+    }
+    // chain of hi: k is the pair (lo, hi) among hi's pairs (x, hi), at position p of sw
+    const int64_t p = key_lower_bound(sw, cnt, ((uint64_t)hi << 32) | lo);
+    int32_t s_hi = -1;
+    if (p > 0 && (uint32_t)(sw[p - 1] >> 32) == hi) npred++;
+    if (p + 1 < cnt && (uint32_t)(sw[p + 1] >> 32) == hi) {
+      const uint64_t nk = ((uint64_t)(uint32_t)sw[p + 1] << 32) | hi;  // the pair (x', hi)
+      s_hi = (int32_t)key_lower_bound(keys, cnt, nk);
+    } else {  // the last pair (x, hi): next comes hi's own run (hi, y), if there is one
+      const int64_t f = key_lower_bound(keys, cnt, (uint64_t)hi << 32);
+      if (f < cnt && (uint32_t)(keys[f] >> 32) == hi) s_hi = (int32_t)f;
+    }
+    succ[2 * k] = s_lo;
+    succ[2 * k + 1] = s_hi;
+    pending[k] = npred;
+  }
+}
+
+__device__ __forceinline__ void sync_all() {
+  __threadfence();
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_resolve_dag(ResolveArgs a) {
+  __shared__ int s_len;
+  __shared__ unsigned long long s_res, s_frag;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const uint32_t found = a.ctr->pair_count;
+  const int cnt = (int)min((int64_t)found, a.pair_cap);
+  if (tid == 0) s_len = 0, s_res = 0, s_frag = 0;
+  __syncthreads();
+  int32_t *cur = a.list_a, *nxt = a.list_b;
+  for (int k = tid; k < cnt; k += nt)
+    if (a.pending[k] == 0) cur[atomicAdd(&s_len, 1)] = k;
+  sync_all();
+  int m = s_len;
+  ResolveTally t;
+  while (m > 0) {
+    __syncthreads();  // every thread has read s_len
+    if (tid == 0) s_len = 0;
+    __syncthreads();
+    for (int q = tid; q < m; q += nt) {
+      const int k = cur[q];
+      const uint64_t key = a.pairs[k];
+      resolve_pair(a, (int)(key >> 32), (int)(key & 0xffffffffu), t);
+      for (int e = 0; e < 2; e++) {
+        const int sk = a.succ[2 * k + e];
+        if (sk >= 0 && atomicSub(&a.pending[sk], 1) == 1) nxt[atomicAdd(&s_len, 1)] = sk;
+      }
+    }
+    sync_all();
+    m = s_len;
+    int32_t *sw = cur;
+    cur = nxt;
+    nxt = sw;
+  }
+  atomicAdd(&s_res, (unsigned long long)t.resolved);
+  atomicAdd(&s_frag, (unsigned long long)t.fragreq);
+  __syncthreads();
+  if (tid == 0) {
+    ResolveTally all;
+    all.resolved = s_res;
+    all.fragreq = s_frag;
+    add_tally(a.ctr, found, all);
+  }
 }
 
 hipError_t launch_resolve(const ResolveArgs &a, hipStream_t st) {
-  hipLaunchKernelGGL(k_resolve, dim3(1), dim3(64), 0, st, a);
+  if (a.events || a.serial || !a.succ) {
+    hipLaunchKernelGGL(k_resolve, dim3(1), dim3(64), 0, st, a);
+    return hipGetLastError();
+  }
+  // the chains: swapped keys, sorted; successors and predecessor counts; then the rounds
+  const unsigned g = (unsigned)std::min<int64_t>(4096, (a.pair_cap + 255) / 256);
+  hipLaunchKernelGGL(k_pairs_swap, dim3(g), dim3(256), 0, st, a.pairs, a.pairs_sw, a.pair_cap,
+                     a.ctr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = launch_sort_pairs(a.pairs_sw, a.sort_tmp, a.pair_cap, a.ctr, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pair_dag, dim3(g), dim3(256), 0, st, a.pairs, a.pairs_sw, a.pair_cap,
+                     a.ctr, a.succ, a.pending);
+  hipLaunchKernelGGL(k_resolve_dag, dim3(1), dim3(1024), 0, st, a);
   return hipGetLastError();
 }
 

@@ -117,8 +117,11 @@ struct Ctx {
   int32_t *fcnt = nullptr, *flist = nullptr;
   int2 *recs = nullptr;
 
-  // collision pairs as 64-bit keys (i << 32 | j)
-  uint64_t *pairs = nullptr;
+  // collision pairs as 64-bit keys (i << 32 | j); the sort's merge buffer and the resolution's
+  // swapped keys (same size); the resolution's per-pair words: 5 int32 per pair (two successors,
+  // a predecessor count, two frontier lists)
+  uint64_t *pairs = nullptr, *pairs_tmp = nullptr, *pairs_sw = nullptr;
+  int32_t *res_words = nullptr;
   int64_t pair_cap = 0;
   int64_t pair_cap_pow2 = 0;
 
@@ -133,6 +136,7 @@ struct Ctx {
   bool shape_tuned = false;
   bool timing = false;
   std::vector<hipEvent_t> ev;
+  std::vector<hipEvent_t> ev_res;  // around each cycle's resolution (timing enabled)
 
   Sur *sur = nullptr;
   bool sur_on = false;
@@ -273,8 +277,9 @@ struct Accel64Args {
 };
 hipError_t launch_accel_f64(const Accel64Args &a, hipStream_t st);
 
-hipError_t launch_sort_pairs(uint64_t *keys, int64_t cap, const DevCounters *ctr,
+hipError_t launch_sort_pairs(uint64_t *keys, uint64_t *tmp, int64_t cap, const DevCounters *ctr,
                              hipStream_t st);
+int sort_pairs(Ctx *c);  // the context's pair list into canonical order
 
 struct ResolveArgs {
   double *x, *y, *z, *vx, *vy, *vz, *m, *r, *rest, *frag;
@@ -289,6 +294,12 @@ struct ResolveArgs {
   FragState *fst;
   const double *acc;  // this cycle's force slices, kept by a body that starts fragmenting
   int split, npad;
+  // events off: pairs that share no body resolve together.  pairs_sw / sort_tmp: pair_cap keys
+  // each (the swapped keys and the sort's merge buffer); succ: 2 per pair; pending, list_a,
+  // list_b: 1 per pair.  serial = one lane in canonical order.
+  uint64_t *pairs_sw, *sort_tmp;
+  int32_t *succ, *pending, *list_a, *list_b;
+  int serial;
 };
 hipError_t launch_resolve(const ResolveArgs &a, hipStream_t st);
 

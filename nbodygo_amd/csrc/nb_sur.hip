@@ -4,9 +4,9 @@
 //
 //   exact near field   FP32 pair sums over the 27 cells adjacent to a body's cell
 //   multipole shell    per-body monopole + quadrupole of every cell at Chebyshev distance 2..K
-//   far expansion      per-cell field, Jacobian and Hessian at the cell's centre of mass from all
-//                      cells beyond distance K (monopole + quadrupole field, monopole derivatives),
-//                      applied to each body by a second-order Taylor step
+//   far expansion      per-cell field and its first three derivatives at the cell's centre of mass
+//                      from all cells beyond distance K (monopole + quadrupole field, monopole
+//                      derivatives), applied to each body by a third-order Taylor step
 //   learned residual   an MLP (BF16 MFMA, nb_mlp.hip) maps scale-free features of the body's cell
 //                      and position to the remaining far-field error, trained on direct-sum data
 //   fallback           every check_every steps the force of check_samples bodies is recomputed by
@@ -39,7 +39,8 @@ int mlp_train_step_device(nb_mlp *m, const float *x_dev, const float *t_dev, int
                           float lr, hipStream_t st);
 
 constexpr int kCellRec = 16;  // COM(3), M, Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, radius, ix, iy, iz, n, 0
-constexpr int kFarRec = 20;   // a(3), J(6: xx yy zz xy xz yz), H(10), 0
+constexpr int kFarRec = 36;   // a(3), J(6: xx yy zz xy xz yz), H(10), T(15), 0, 0
+constexpr int kFarUsed = 34;  // words of the record the far-field kernel writes
 constexpr int kFeat = 30;     // MLP input width
 constexpr int kMaxGrid = 64;  // cell keys 0..64^3 (the last for inactive bodies): 19 bits
 // Bits of the cell keys 0..ncell sorted by the radix sort (16 up to 40^3 cells).
@@ -75,6 +76,8 @@ struct Sur {
   int32_t *sidx = nullptr;
   int S = 0;
   double *sexact = nullptr, *serr = nullptr, *spart = nullptr;
+  uint32_t *sbody = nullptr;  // per-sample relative error (float bits) of the last check
+  uint64_t draws = 0;         // sample draws so far (rotates the samples)
   int64_t spart_len = 0;
   float *ymlp = nullptr;  // MLP output [cap x 3]
   uint32_t *flag = nullptr;
@@ -376,19 +379,36 @@ __global__ __launch_bounds__(64) void k_sur_moments(const float4 *sp, const int3
 // H_abc = 3 M [(d_ab r_c + d_ac r_b + d_bc r_a) / rs^5 - 5 r_a r_b r_c / rs^7], plus the
 // quadrupole field Q r / rs^5 - 5/2 (r.Q.r) r / rs^7; r = X_target - X_source.
 //
-// Work unit = (64 target cells, one lane each) x (kM2LSlice source cells); the four waves of a
-// workgroup take a quarter of the slice each, staging 64 source records at a time in a private
-// LDS tile (one record per lane) and reading them back as wave-uniform broadcasts, so every VALU
-// operand is a VGPR; the waves are summed in LDS in wave order.  Partials per slice go to
-// `part`, summed by k_sur_m2l_sum.
-constexpr int kM2LSlice = 1024;
-
+// Work unit = (64 target cells, one lane each) x (one slice of the source cells); the four waves
+// of a workgroup take a quarter of the slice each, staging 64 source records at a time in a
+// private LDS tile (one record per lane) and reading them back as wave-uniform broadcasts, so
+// every VALU operand is a VGPR; the waves are summed in LDS in wave order.  The slice count is
+// chosen from the nonempty-cell count so that there are about kM2LUnits work units; partials per
+// slice go to `part` ([slice][nonempty cell][kFarRec]), summed by k_sur_m2l_sum.  `part` then
+// never exceeds (kM2LUnits * 64 + ncell) * kFarRec floats.
+constexpr int kM2LUnits = 4096;
+struct M2LSlices {
+  int ntg, nsl, width;  // target groups, slices, source cells per slice (a multiple of 256)
+};
+__host__ __device__ inline M2LSlices m2l_slices(int nec) {
+  M2LSlices r;
+  r.ntg = (nec + 63) / 64;
+  int nsl = r.ntg > 0 ? (kM2LUnits + r.ntg - 1) / r.ntg : 1;
+  const int most = (nec + 255) / 256;
+  nsl = nsl < most ? nsl : most;
+  nsl = nsl > 1 ? nsl : 1;
+  r.width = ((nec + nsl - 1) / nsl + 255) / 256 * 256;
+  r.nsl = r.width > 0 ? (nec + r.width - 1) / r.width : 1;
+  return r;
+}
+inline size_t m2l_part_floats(int ncell) { return ((size_t)kM2LUnits * 64 + ncell) * kFarRec; }
 __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int32_t *necount,
                                                  int K, float eps2_s, int ncell, float *part) {
-  __shared__ float red[3][kFarRec - 1][64];
+  __shared__ float red[3][kFarUsed][64];
   __shared__ float4 recs[4][64][4];
   const int nec = *necount;
-  const int ntg = (nec + 63) / 64, nsl = (nec + kM2LSlice - 1) / kM2LSlice;
+  const M2LSlices ms = m2l_slices(nec);
+  const int ntg = ms.ntg, nsl = ms.nsl;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const float eps2 = to_vgpr(eps2_s);
@@ -401,11 +421,11 @@ __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int
     const bool valid = t < nec;
     const float *tr = crec_ne + (size_t)(valid ? t : nec - 1) * kCellRec;
     const float X = tr[0], Y = tr[1], Z = tr[2], ix = tr[11], iy = tr[12], iz = tr[13];
-    float o[kFarRec - 1];
+    float o[kFarUsed];
 #pragma unroll
-    for (int q = 0; q < kFarRec - 1; q++) o[q] = 0.f;
-    const int per = kM2LSlice / 4;
-    const int s0 = sl * kM2LSlice + wave * per, s1 = min(s0 + per, nec);
+    for (int q = 0; q < kFarUsed; q++) o[q] = 0.f;
+    const int per = ms.width / 4;
+    const int s0 = sl * ms.width + wave * per, s1 = min(s0 + per, nec);
     for (int sb = s0; sb < s1; sb += 64) {
       const int cnt = min(64, s1 - sb);
       // padding records are all zero: no mass, no quadrupole, no contribution
@@ -460,17 +480,37 @@ __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int
         o[16] += az5 - f5 * ry * ry * rz;
         o[17] += ay5 - f5 * ry * rz * rz;
         o[18] += 3.f * az5 - f5 * rz * rz * rz;
+        // third derivatives of the monopole field (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz
+        // xzzz yyyy yyyz yyzz yzzz zzzz): 3 m5 (dd + dd + dd) - 15 m7 (six d r r) + 105 m9 rrrr
+        const float g9 = 105.f * m7 * inv2, xx = rx * rx, yy = ry * ry, zz = rz * rz;
+        const float xy = rx * ry, xz = rx * rz, yz = ry * rz;
+        const float u45 = 3.f * f5, u9 = 3.f * t3;  // 45 m7, 9 m5
+        o[19] += u9 - 6.f * f5 * xx + g9 * xx * xx;
+        o[20] += -u45 * xy + g9 * xx * xy;
+        o[21] += -u45 * xz + g9 * xx * xz;
+        o[22] += t3 - f5 * (xx + yy) + g9 * xx * yy;
+        o[23] += -f5 * yz + g9 * xx * yz;
+        o[24] += t3 - f5 * (xx + zz) + g9 * xx * zz;
+        o[25] += -u45 * xy + g9 * xy * yy;
+        o[26] += -f5 * xz + g9 * xz * yy;
+        o[27] += -f5 * xy + g9 * xy * zz;
+        o[28] += -u45 * xz + g9 * xz * zz;
+        o[29] += u9 - 6.f * f5 * yy + g9 * yy * yy;
+        o[30] += -u45 * yz + g9 * yy * yz;
+        o[31] += t3 - f5 * (yy + zz) + g9 * yy * zz;
+        o[32] += -u45 * yz + g9 * yz * zz;
+        o[33] += u9 - 6.f * f5 * zz + g9 * zz * zz;
       }
       wave_lds_sync();
     }
     if (wave > 0)
-      for (int q = 0; q < kFarRec - 1; q++) red[wave - 1][q][lane] = o[q];
+      for (int q = 0; q < kFarUsed; q++) red[wave - 1][q][lane] = o[q];
     __syncthreads();
     if (wave == 0 && valid) {
-      float *dst = part + ((size_t)sl * ncell + t) * kFarRec;
-      for (int q = 0; q < kFarRec - 1; q++)
+      float *dst = part + ((size_t)sl * nec + t) * kFarRec;
+      for (int q = 0; q < kFarUsed; q++)
         dst[q] = ((o[q] + red[0][q][lane]) + red[1][q][lane]) + red[2][q][lane];
-      dst[kFarRec - 1] = 0.f;
+      for (int q = kFarUsed; q < kFarRec; q++) dst[q] = 0.f;
     }
     __syncthreads();
   }
@@ -483,9 +523,9 @@ __global__ void k_sur_m2l_sum(const float *part, const int32_t *nelist, const in
   const int nec = *necount;
   if (g >= nec * kFarRec) return;
   const int t = g / kFarRec, q = g % kFarRec;
-  const int nsl = (nec + kM2LSlice - 1) / kM2LSlice;
+  const int nsl = m2l_slices(nec).nsl;
   float s = 0.f;
-  for (int sl = 0; sl < nsl; sl++) s += part[(size_t)sl * ncell * kFarRec + g];
+  for (int sl = 0; sl < nsl; sl++) s += part[(size_t)sl * nec * kFarRec + g];
   fexp[(size_t)nelist[t] * kFarRec + q] = s;
 }
 
@@ -795,7 +835,7 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   ax = ((ax + red[0][0][lane]) + red[1][0][lane]) + red[2][0][lane];
   ay = ((ay + red[0][1][lane]) + red[1][1][lane]) + red[2][1][lane];
   az = ((az + red[0][2][lane]) + red[1][2][lane]) + red[2][2][lane];
-  // second-order Taylor step of the far expansion from the cell's centre of mass
+  // third-order Taylor step of the far expansion from the cell's centre of mass
   cfp cr = (cfp)(a.crec + (size_t)c * kCellRec);
   cfp fe = (cfp)(a.fexp + (size_t)c * kFarRec);
   const float dx = me.x - cr[0], dy = me.y - cr[1], dz = me.z - cr[2];
@@ -811,6 +851,29 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
                 2.f * (H[3] * dx * dy + H[4] * dx * dz + H[7] * dy * dz));
   tz += 0.5f * (H[2] * dx * dx + H[7] * dy * dy + H[9] * dz * dz +
                 2.f * (H[4] * dx * dy + H[5] * dx * dz + H[8] * dy * dz));
+  // 1/6 T_abcd d_b d_c d_d with T index (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz xzzz yyyy
+  // yyyz yyzz yzzz zzzz); for component a the sum over (b, c, d) counts each index multiset by
+  // its number of orderings
+  cfp T = fe + 19;
+  {
+    const float x2 = dx * dx, y2 = dy * dy, z2 = dz * dz;
+    const float x3 = x2 * dx, y3 = y2 * dy, z3 = z2 * dz;
+    const float x2y = x2 * dy, x2z = x2 * dz, xy2 = dx * y2, xz2 = dx * z2, y2z = y2 * dz,
+                yz2 = dy * z2, xyz = dx * dy * dz;
+    const float c6 = 1.f / 6.f;
+    // component x: T_x... = (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz xzzz)
+    tx += c6 * (T[0] * x3 + 3.f * (T[1] * x2y + T[2] * x2z + T[3] * xy2 + T[5] * xz2 +
+                                   T[7] * y2z + T[8] * yz2) +
+                6.f * T[4] * xyz + T[6] * y3 + T[9] * z3);
+    // component y: T_y... = (xxxy xxyy xxyz xyyy xyyz xyzz yyyy yyyz yyzz yzzz)
+    ty += c6 * (T[1] * x3 + 3.f * (T[3] * x2y + T[4] * x2z + T[6] * xy2 + T[8] * xz2 +
+                                   T[11] * y2z + T[12] * yz2) +
+                6.f * T[7] * xyz + T[10] * y3 + T[13] * z3);
+    // component z: T_z... = (xxxz xxyz xxzz xyyz xyzz xzzz yyyz yyzz yzzz zzzz)
+    tz += c6 * (T[2] * x3 + 3.f * (T[4] * x2y + T[5] * x2z + T[7] * xy2 + T[9] * xz2 +
+                                   T[12] * y2z + T[13] * yz2) +
+                6.f * T[8] * xyz + T[11] * y3 + T[14] * z3);
+  }
   ax += tx;
   ay += ty;
   az += tz;
@@ -821,7 +884,10 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   for (int q = 0; q < 6; q++) nj += J[q] * J[q] * (q < 3 ? 1.f : 2.f);
   const float hw[10] = {1, 3, 3, 3, 6, 3, 1, 3, 3, 1};
   for (int q = 0; q < 10; q++) nh += hw[q] * H[q] * H[q];
-  const float A = na + sqrtf(nj) * sc + sqrtf(nh) * sc * sc + 1e-30f;
+  float nt = 0;
+  const float tw[15] = {1, 4, 4, 6, 12, 6, 4, 12, 12, 4, 1, 4, 6, 4, 1};
+  for (int q = 0; q < 15; q++) nt += tw[q] * T[q] * T[q];
+  const float A = na + sqrtf(nj) * sc + sqrtf(nh) * sc * sc + sqrtf(nt) * sc * sc * sc + 1e-30f;
   a.base[k] = make_float4((float)ax, (float)ay, (float)az, A);
   if (a.write_feat) {
     float f[kFeat];
@@ -917,29 +983,48 @@ __global__ __launch_bounds__(256) void k_sur_exact_sum(const double *part, int S
   if (lane == 0) exact[g] = v;
 }
 
-// Relative error of the surrogate over the samples (fixed-order tree reduction); raises the
-// fallback flag above tol.
+// Relative error of the surrogate over the samples: the global ||da|| / ||a|| (fixed-order tree
+// reduction) and each sample's own |da_s| / |a_s|, of which the k-th largest is found by a radix
+// select on the float bits (exact; k = floor((1 - quantile) S) + 1, so at most a fraction
+// 1 - quantile of the samples lies above it).  The fallback flag is raised when the global error
+// exceeds tol or that per-body value exceeds tol_body (tol_body > 0).  err[0] = global, err[1] =
+// the per-body value.
 __global__ __launch_bounds__(256) void k_sur_check(const int32_t *sidx, int S,
                                                    const uint32_t *perm, const int32_t *cstart,
                                                    int ncell, const double *exact,
                                                    const double *acc, int npad, double tol,
-                                                   uint32_t *flag, double *err) {
+                                                   double tol_body, double quantile,
+                                                   uint32_t *body, uint32_t *flag, double *err) {
   __shared__ double sn[256], sd[256];
+  __shared__ int hist[256];
+  __shared__ uint32_t s_prefix;
+  __shared__ int s_k;
   const int nact = cstart[ncell];
   double num = 0, den = 0;
   for (int s = threadIdx.x; s < S; s += 256) {
     const int k = sidx[s];
-    if (k >= nact) continue;
-    const uint32_t i = perm[k];
-    for (int q = 0; q < 3; q++) {
-      const double e = exact[(size_t)s * 3 + q];
-      const double d = acc[(size_t)q * npad + i] - e;
-      num += d * d;
-      den += e * e;
+    double bn = 0, bd = 0;
+    if (k < nact) {
+      const uint32_t i = perm[k];
+      for (int q = 0; q < 3; q++) {
+        const double e = exact[(size_t)s * 3 + q];
+        const double d = acc[(size_t)q * npad + i] - e;
+        bn += d * d;
+        bd += e * e;
+      }
     }
+    num += bn;
+    den += bd;
+    const float r = bd > 0 ? (float)sqrt(bn / bd) : (bn > 0 ? INFINITY : 0.f);
+    body[s] = r != r ? 0x7fc00000u : __float_as_uint(r);  // NaN ranks above every number
   }
   sn[threadIdx.x] = num;
   sd[threadIdx.x] = den;
+  if (threadIdx.x == 0) {
+    s_prefix = 0;
+    const double above = floor((1.0 - quantile) * (double)S);
+    s_k = (int)(above < 0 ? 0 : (above > S - 1 ? S - 1 : above)) + 1;
+  }
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
@@ -948,10 +1033,34 @@ __global__ __launch_bounds__(256) void k_sur_check(const int32_t *sidx, int S,
     }
     __syncthreads();
   }
+  uint32_t mask = 0;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t prefix = s_prefix;
+    for (int s = threadIdx.x; s < S; s += 256) {
+      const uint32_t u = body[s];
+      if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // the digit of the k-th largest among the values with this prefix
+      int k = s_k, b = 255;
+      for (; b > 0; b--) {
+        if (hist[b] >= k) break;
+        k -= hist[b];
+      }
+      s_k = k;
+      s_prefix = prefix | ((uint32_t)b << shift);
+    }
+    mask |= 255u << shift;
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     const double r = sd[0] > 0 ? sqrt(sn[0] / sd[0]) : 0.0;
-    *err = r;
-    *flag = r > tol ? 1u : 0u;
+    const double rb = (double)__uint_as_float(s_prefix);
+    err[0] = r;
+    err[1] = rb;
+    *flag = (r > tol || (tol_body > 0 && !(rb <= tol_body))) ? 1u : 0u;
   }
 }
 
@@ -1009,11 +1118,27 @@ inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 // ---------------------------------------------------------------------------------------------
 namespace nb {
 
-__global__ void k_sur_samples(int32_t *sidx, int S, const int32_t *cstart, int ncell) {
+// Sample s of a check: rank floor((s + u_s) nact / S) of the cell-sorted order, with u_s in [0, 1)
+// from a counter-based hash of (seed, draw, s): one body from each of S equal strata of the
+// sorted order (every region of space in proportion to its bodies), a different one at each
+// draw.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_sur_samples(int32_t *sidx, int S, const int32_t *cstart, int ncell,
+                              uint64_t seed, uint64_t draw) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   const int nact = cstart[ncell];
-  sidx[s] = nact > 0 ? (int)(((int64_t)s * nact) / S) : 0;
+  const uint64_t h = mix64(seed ^ mix64(draw * 0x2545f4914f6cdd1dull + (uint64_t)s));
+  const double u = (double)(h >> 11) * 0x1.0p-53;
+  int64_t k = (int64_t)(((double)s + u) * (double)nact / (double)S);
+  if (k >= nact) k = nact - 1;
+  sidx[s] = nact > 0 ? (int)k : 0;
 }
 
 __global__ void k_sur_count_fallback(const uint32_t *flag, DevCounters *ctr) {
@@ -1035,6 +1160,7 @@ void sur_free(Sur *s) {
                   s->nelist, s->necount, s->sp, s->crec, s->crec_ne, s->fexp, s->part,
                   s->items, s->nitems, s->btot, s->feat, s->base, s->center,
                   s->red, s->mu, s->sd, s->tx, s->ty, s->sidx, s->sexact, s->serr, s->flag,
+                  s->sbody,
                   s->spart, s->ymlp,
                   s->mu0, s->sd1, s->near_pairs};
   for (void *q : ptrs)
@@ -1078,6 +1204,7 @@ static int sur_ensure_samples(Ctx *c, int S) {
   if (S > s->S) {
     NB_HIP(c, salloc(&s->sidx, (size_t)S));
     NB_HIP(c, salloc(&s->sexact, (size_t)S * 3));
+    NB_HIP(c, salloc(&s->sbody, (size_t)S));
     s->S = S;
   }
   const int64_t need = (c->cap + kExactSlice - 1) / kExactSlice * (int64_t)s->S * 3;
@@ -1088,22 +1215,26 @@ static int sur_ensure_samples(Ctx *c, int S) {
   return NB_OK;
 }
 
-// Sampled error of the accelerations in acc slice 0 against the FP32 direct sum: S bodies evenly
-// spread over the sorted order; the error goes to serr and, above tol, the fallback flag is set.
-static int sur_check_launch(Ctx *c, int S, double tol) {
+// Sampled error of the accelerations in acc slice 0 against the FP32 direct sum: S bodies, one
+// per stratum of the sorted order, drawn afresh at every check (rotate) or the same fixed draw
+// (a measurement that callers compare across settings); the global and per-body errors go to
+// serr and, above tol or tol_body, the fallback flag is set.
+static int sur_check_launch(Ctx *c, int S, double tol, double tol_body, bool rotate) {
   Sur *s = c->sur;
   hipStream_t st = c->stream;
   int rc = sur_ensure_samples(c, S);
   if (rc) return rc;
+  const uint64_t draw = rotate ? (uint64_t)s->draws++ : ~0ull;
   hipLaunchKernelGGL(k_sur_samples, dim3(nb256(S)), dim3(256), 0, st, s->sidx, S, s->cstart,
-                     s->ncell);
+                     s->ncell, (uint64_t)s->p.seed, draw);
   const unsigned nsl = (unsigned)((c->n + kExactSlice - 1) / kExactSlice);
   hipLaunchKernelGGL(k_sur_exact, dim3((unsigned)((S + 63) / 64), nsl), dim3(64), 0, st, s->sp,
                      s->cstart, s->ncell, s->sidx, S, (float)(c->p.eps * c->p.eps), s->spart);
   hipLaunchKernelGGL(k_sur_exact_sum, dim3((unsigned)((S * 3 + 3) / 4)), dim3(256), 0, st,
                      s->spart, S, s->cstart, s->ncell, s->sexact);
   hipLaunchKernelGGL(k_sur_check, dim3(1), dim3(256), 0, st, s->sidx, S, s->perm, s->cstart,
-                     s->ncell, s->sexact, c->acc, (int)c->npad, tol, s->flag, s->serr);
+                     s->ncell, s->sexact, c->acc, (int)c->npad, tol, tol_body,
+                     s->p.body_quantile, s->sbody, s->flag, s->serr);
   NB_HIP(c, hipGetLastError());
   return NB_OK;
 }
@@ -1204,7 +1335,7 @@ int sur_force_phase(Ctx *c, bool timed, int ev_slot) {
   if (rc) return rc;
   if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], st));
   if (s->p.check_every > 0 && s->steps % s->p.check_every == 0 && c->n > 0) {
-    rc = sur_check_launch(c, s->p.check_samples, s->p.tol);
+    rc = sur_check_launch(c, s->p.check_samples, s->p.tol, s->p.tol_body, true);
     if (rc) return rc;
     hipLaunchKernelGGL(k_sur_count_fallback, dim3(1), dim3(64), 0, st, s->flag, c->ctr);
     NB_HIP(c, hipGetLastError());
@@ -1219,7 +1350,10 @@ int sur_force_phase(Ctx *c, bool timed, int ev_slot) {
     NB_HIP(c, hipGetLastError());
     s->checks++;
   }
-  if (detect) NB_HIP(c, launch_sort_pairs(c->pairs, c->pair_cap, c->ctr, st));
+  if (detect) {
+    const int rcs = sort_pairs(c);
+    if (rcs) return rcs;
+  }
   s->steps++;
   return NB_OK;
 }
@@ -1241,6 +1375,8 @@ int nb_sur_params_init(nb_sur_params *p) {
   p->check_samples = 256;
   p->tol = 1e-3;
   p->seed = 12345;
+  p->tol_body = 0.0;
+  p->body_quantile = 0.99;
   return NB_OK;
 }
 
@@ -1253,7 +1389,8 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     return fail_msg(c, NB_E_UNSUPPORTED, "the surrogate path has no overlap cutoff");
   if (p->grid < 4 || p->grid > kMaxGrid || p->shell < 1 || p->shell > 6 || p->check_every < 0 ||
       (p->check_every > 0 && (p->check_samples < 1 || p->check_samples > 65536)) ||
-      !(p->tol > 0) || p->r0 < 0)
+      !(p->tol > 0) || p->r0 < 0 || !(p->tol_body >= 0) ||
+      !(p->body_quantile > 0 && p->body_quantile <= 1))
     return fail_msg(c, NB_E_INVALID, "nb_sur_enable: bad parameters");
   NB_HIP(c, hipSetDevice(c->device));
   Sur *s = c->sur;
@@ -1278,7 +1415,7 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     NB_HIP(c, salloc(&s->crec_ne, nc * kCellRec));
     NB_HIP(c, salloc(&s->btot, nc / 1024 + 2));
     NB_HIP(c, salloc(&s->fexp, nc * kFarRec));
-    NB_HIP(c, salloc(&s->part, nc * kFarRec * (size_t)((nc + kM2LSlice - 1) / kM2LSlice)));
+    NB_HIP(c, salloc(&s->part, m2l_part_floats((int)nc)));
     s->items_cap = 0;  // re-sized for the new grid by sur_ensure_bodies
   }
   if (fresh) {
@@ -1288,7 +1425,7 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     NB_HIP(c, salloc(&s->sd, kFeat));
     NB_HIP(c, salloc(&s->mu0, kFeat));
     NB_HIP(c, salloc(&s->sd1, kFeat));
-    NB_HIP(c, salloc(&s->serr, 1));
+    NB_HIP(c, salloc(&s->serr, 2));
     NB_HIP(c, salloc(&s->flag, 1));
     NB_HIP(c, salloc(&s->near_pairs, 1));
     NB_HIP(c, salloc(&s->nitems, 1));
@@ -1297,7 +1434,7 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     NB_HIP(c, hipMemcpy(s->sd1, o.data(), kFeat * 4, hipMemcpyHostToDevice));
     NB_HIP(c, hipMemcpy(s->mu, z.data(), kFeat * 4, hipMemcpyHostToDevice));
     NB_HIP(c, hipMemcpy(s->sd, o.data(), kFeat * 4, hipMemcpyHostToDevice));
-    NB_HIP(c, hipMemset(s->serr, 0, sizeof(double)));
+    NB_HIP(c, hipMemset(s->serr, 0, 2 * sizeof(double)));
     NB_HIP(c, hipMemset(s->flag, 0, sizeof(uint32_t)));
     NB_HIP(c, hipMemset(s->near_pairs, 0, sizeof(double)));
   }
@@ -1489,7 +1626,7 @@ int nb_sur_error(nb_ctx *c, int32_t samples, int32_t use_mlp, double *err) {
   const bool mlp = use_mlp && s->trained;
   int rc = sur_pipeline(c, false, mlp, true, mlp);
   if (rc) return rc;
-  rc = sur_check_launch(c, samples, 1e300);
+  rc = sur_check_launch(c, samples, 1e300, 0.0, false);
   if (rc) return rc;
   NB_HIP(c, hipMemcpyAsync(err, s->serr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   NB_HIP(c, hipStreamSynchronize(c->stream));
@@ -1503,10 +1640,10 @@ int nb_sur_get_stats(nb_ctx *c, nb_sur_stats *out) {
   Sur *s = c->sur;
   NB_HIP(c, hipSetDevice(c->device));
   DevCounters ctr;
-  double err = 0, pairs = 0;
+  double err[2] = {0, 0}, pairs = 0;
   int32_t nec = 0;
   NB_HIP(c, hipMemcpyAsync(&ctr, c->ctr, sizeof ctr, hipMemcpyDeviceToHost, c->stream));
-  NB_HIP(c, hipMemcpyAsync(&err, s->serr, sizeof err, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(c, hipMemcpyAsync(err, s->serr, sizeof err, hipMemcpyDeviceToHost, c->stream));
   NB_HIP(c, hipMemcpyAsync(&pairs, s->near_pairs, sizeof pairs, hipMemcpyDeviceToHost,
                            c->stream));
   NB_HIP(c, hipMemcpyAsync(&nec, s->necount, 4, hipMemcpyDeviceToHost, c->stream));
@@ -1514,7 +1651,8 @@ int nb_sur_get_stats(nb_ctx *c, nb_sur_stats *out) {
   out->steps = s->steps;
   out->checks = s->checks;
   out->fallbacks = (int64_t)ctr.sur_fallbacks;
-  out->last_error = err;
+  out->last_error = err[0];
+  out->last_body_error = err[1];
   out->near_pairs_total = pairs;
   out->nonempty_cells = nec;
   out->train_rows = s->trows;

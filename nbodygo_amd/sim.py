@@ -84,7 +84,7 @@ class NBodyGPU:
 
     def __init__(self, G=6.673e-11, eps=0.0, overlap_cutoff=True, collisions=True,
                  precision="fp32", device=0, pair_capacity=0, ipl=0, tile_j=0, jsplit=0,
-                 block=0, variant=0, events=False):
+                 block=0, variant=0, events=False, resolve=0):
         p = L.NbParams()
         check(lib.nb_params_init(C.byref(p)))
         p.G = float(G)
@@ -96,6 +96,7 @@ class NBodyGPU:
         p.pair_capacity = int(pair_capacity)
         p.ipl, p.tile_j, p.jsplit, p.block, p.variant = ipl, tile_j, jsplit, block, variant
         p.events = int(bool(events))
+        p.resolve = int(resolve)
         self.params = p
         self._ctx = C.c_void_p()
         check(lib.nb_create(C.byref(p), C.byref(self._ctx)), None)
@@ -174,11 +175,15 @@ class NBodyGPU:
         return Bodies(*f, beh=beh, flags=flags, rest=rest, frag=frag, fstep=fstep)
 
     # -- Barnes-Hut octree (cmd/body/octree.go with Build wired in) ---------------------------
-    def bh_enable(self, theta=0.3, softening=1e-4, max_depth=21, group=0):
+    def bh_enable(self, theta=0.3, softening=1e-4, max_depth=21, group=0, mode="exact"):
+        """The reference's octree force (FP64 contexts).  mode "exact": per-body walks with the
+        reference's test and FP64 terms in its order; "fast": one walk per 64-body group that
+        opens what any of them would, FP32 terms."""
         p = L.NbBhParams()
         check(lib.nb_bh_params_init(C.byref(p)))
         p.theta, p.softening, p.max_depth = float(theta), float(softening), int(max_depth)
         p.group = int(group)
+        p.mode = {"exact": 0, "fast": 1}[mode] if isinstance(mode, str) else int(mode)
         self._check(lib.nb_bh_enable(self._ctx, C.byref(p)))
 
     def bh_disable(self):
@@ -266,15 +271,18 @@ class NBodyGPU:
 
     # ---- learned far-field surrogate ----------------------------------------------------------
     def sur_enable(self, grid=32, shell=2, r0=0.0, use_mlp=True, check_every=1,
-                   check_samples=256, tol=1e-3, seed=12345):
+                   check_samples=256, tol=1e-3, seed=12345, tol_body=0.0, body_quantile=0.99):
         """Replace the O(N^2) force phase by the cell-grid multipole + MLP surrogate (FP32 path,
         eps > 0, no overlap cutoff; with collisions every radius must be <= r0 / grid).  Steps
-        whose sampled error exceeds tol are redone by direct summation."""
+        whose sampled error exceeds tol, or (tol_body > 0) whose per-body errors exceed tol_body
+        on more than a fraction 1 - body_quantile of the samples, are redone by direct
+        summation."""
         p = L.NbSurParams()
         lib.nb_sur_params_init(C.byref(p))
         p.grid, p.shell, p.r0 = int(grid), int(shell), float(r0)
         p.use_mlp, p.check_every = int(bool(use_mlp)), int(check_every)
         p.check_samples, p.tol, p.seed = int(check_samples), float(tol), int(seed)
+        p.tol_body, p.body_quantile = float(tol_body), float(body_quantile)
         self._check(lib.nb_sur_enable(self._ctx, C.byref(p)))
         return self
 
@@ -386,6 +394,17 @@ class NBodyBatch:
         self._check(lib.nb_batch_set_shapes(self._b, s))
 
     def autotune(self, reps=2):
+        """Tune waves per item, and IPL and Kahan tile per size bucket; returns the tuning."""
         s = (C.c_int32 * 3)()
         self._check(lib.nb_batch_autotune(self._b, int(reps), s))
-        return list(s)
+        return self.tuning()
+
+    def tuning(self):
+        """{"waves": W, "buckets": [(ipl, tile) for N < 1024, < 4096, >= 4096]}"""
+        c = (C.c_int32 * 7)()
+        self._check(lib.nb_batch_get_tuning(self._b, c))
+        return {"waves": c[0], "buckets": [(c[1 + 2 * k], c[2 + 2 * k]) for k in range(3)]}
+
+    def set_tuning(self, waves, buckets):
+        c = (C.c_int32 * 7)(int(waves), *[int(v) for ipl_tile in buckets for v in ipl_tile])
+        self._check(lib.nb_batch_set_tuning(self._b, c))

@@ -2,7 +2,8 @@
 
 Bars: per-simulation forces within FP32 error of the FP64 oracle (global <= 1e-5); simulations
 never see each other; the hipGraph launch is bit-identical to eager launches of the same kernels,
-replay after replay; the target-block width (IPL) does not change any body's arithmetic."""
+replay after replay; the target-block width (IPL) does not change any body's arithmetic, and
+every (waves, tile) tuning meets the same oracle bar."""
 import numpy as np
 import pytest
 
@@ -21,10 +22,12 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def test_batch_forces_match_oracle_per_sim(oracle):
+@pytest.mark.parametrize("waves,tile", [(4, 64), (1, 256), (2, 128)])
+def test_batch_forces_match_oracle_per_sim(oracle, waves, tile):
     sims = sims_mixed()
     with NBodyBatch(G=1.0, eps=0.01) as bt:
         bt.load(sims)
+        bt.set_tuning(waves, [(2, tile), (4, tile), (4, tile)])
         f = bt.forces()
     off = bt.offsets
     for k, s in enumerate(sims):
@@ -64,6 +67,9 @@ def test_graph_replays_match_eager_launches():
 
 
 def test_shapes_do_not_change_results():
+    """IPL only regroups targets: forces are bit-identical across it for a given (waves, tile).
+    Waves and tile change where the Kahan tiles and the source split fall, so forces agree to
+    FP32 rounding (1e-6 relative), and every tuning gives the oracle's forces to the FP32 bar."""
     sims = sims_mixed(seed=5)
     out = []
     for ipl in (1, 2, 4):
@@ -74,11 +80,20 @@ def test_shapes_do_not_change_results():
             out.append(bt.forces())
     np.testing.assert_array_equal(out[0], out[1])
     np.testing.assert_array_equal(out[0], out[2])
+    for waves in (1, 2, 4):
+        for tile in (64, 128, 256):
+            with NBodyBatch(G=1.0, eps=0.01) as bt:
+                bt.load(sims)
+                bt.set_tuning(waves, [(2, tile), (4, tile), (1, tile)])
+                assert bt.tuning() == {"waves": waves, "buckets": [(2, tile), (4, tile), (1, tile)]}
+                f = bt.forces()
+            assert rel(f, out[0]) < 1e-6, (waves, tile)
     with NBodyBatch(G=1.0, eps=0.01) as bt:
         bt.load(sims)
         tuned = bt.autotune(reps=1)
-        assert all(q in (1, 2, 4) for q in tuned)
-        np.testing.assert_array_equal(bt.forces(), out[0])
+        assert tuned["waves"] in (1, 2, 4)
+        assert all(i in (1, 2, 4) and t in (64, 128, 256) for i, t in tuned["buckets"])
+        assert rel(bt.forces(), out[0]) < 1e-6
 
 
 def test_batch_step_tracks_single_context():

@@ -178,3 +178,76 @@ def test_tree_after_a_device_cycle_matches_a_fresh_context():
         h.bh_enable()
         np.testing.assert_array_equal(h.forces(), f)
         np.testing.assert_array_equal(h.bh_depths(), depth)
+
+
+# ---------------------------------------------------------------------------------------------
+# Fast mode: one walk per 64-body group (opening what any of them would open), FP32 terms
+# ---------------------------------------------------------------------------------------------
+def run_modes(b, theta=0.3, max_depth=21):
+    out = {}
+    with NBodyGPU(G=1.0, precision="fp64", collisions=False) as g:
+        g.load(b)
+        for mode in ("fast", "exact"):
+            g.bh_enable(theta=theta, max_depth=max_depth, mode=mode)
+            out[mode] = (g.forces(), g.bh_stats())
+    return out
+
+
+def glob(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("kind,n", [("plummer", 65536), ("two_plummer", 30000),
+                                    ("uniform_sphere", 30000)])
+def test_fast_mode_against_the_fp64_direct_sum(oracle, kind, n):
+    """Error against the FP64 softened direct sum (octree.go:314-325 over every body) not above
+    exact mode's: the group test opens at least what each body's own test opens.  On the
+    Plummer sphere, within 2.5e-4.  Removed and fragmenting bodies feel nothing."""
+    b = mixed(getattr(gen, kind)(n, seed=7), 7)
+    st = oracle.State.from_bodies(b)
+    r = run_modes(b)
+    idx = np.arange(0, b.n, 31)
+    ref = oracle.force_soft(st, 1.0, 1e-4, idx=idx, nthreads=16)
+    act = ((b.flags & L.NB_F_EXISTS) != 0) & ((b.flags & L.NB_F_FRAGMENTING) == 0)
+    sel = act[idx]
+    e_fast = glob(r["fast"][0][idx][sel], ref[sel])
+    e_exact = glob(r["exact"][0][idx][sel], ref[sel])
+    print("%s N=%d: fast %.3e, exact %.3e, terms fast/exact %.2f" % (
+        kind, n, e_fast, e_exact, r["fast"][1]["interactions"] / r["exact"][1]["interactions"]),
+        flush=True)
+    if kind == "plummer":
+        assert e_fast <= 2.5e-4
+    assert e_fast <= e_exact * 1.05 + 2e-6
+    assert np.all(r["fast"][0][~act] == 0)
+    assert r["fast"][1]["overflow"] == 0
+
+
+def test_fast_mode_theta_zero_is_an_fp32_direct_sum(oracle):
+    b = mixed(gen.plummer(20000, seed=8), 8)
+    st = oracle.State.from_bodies(b)
+    r = run_modes(b, theta=0.0)
+    ref = oracle.force_soft(st, 1.0, 1e-4, nthreads=16)
+    act = ((b.flags & L.NB_F_EXISTS) != 0) & ((b.flags & L.NB_F_FRAGMENTING) == 0)
+    assert glob(r["fast"][0][act], ref[act]) <= 1e-5
+    # every other active body is one term of every active body
+    na = int(act.sum())
+    assert r["fast"][1]["interactions"] == na * na
+
+
+def test_fast_mode_suns_and_buckets(oracle):
+    """Suns keep the exact direct sums (bit-identical to exact mode); coincident groups under a
+    low depth limit are buckets whose bodies are terms one by one."""
+    b = gen.plummer(6000, seed=9)
+    for k in range(1, 6):
+        b.x[k * 100:k * 100 + k + 1] = b.x[k * 100]
+        b.y[k * 100:k * 100 + k + 1] = b.y[k * 100]
+        b.z[k * 100:k * 100 + k + 1] = b.z[k * 100]
+    b.flags[[1, 99, 5999]] |= L.NB_F_SUN
+    b.m[[1, 99, 5999]] *= 1e3
+    st = oracle.State.from_bodies(b)
+    ref = oracle.force_soft(st, 1.0, 1e-4, nthreads=16)
+    for max_depth in (21, 5):
+        r = run_modes(b, max_depth=max_depth)
+        np.testing.assert_array_equal(r["fast"][0][[1, 99, 5999]], r["exact"][0][[1, 99, 5999]])
+        assert r["fast"][1]["buckets"] > 0
+        assert glob(r["fast"][0], ref) <= 2.5e-4

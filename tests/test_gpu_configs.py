@@ -1,0 +1,108 @@
+"""The benchmark configurations at their stated sizes, through the C-ABI, against the CPU oracle.
+
+config 4   Plummer sphere of N = 262,144 (a = 1, M = 1, G = 1, eps = 0.01): the learned far-field
+           surrogate (32^3 stretched grid, shell 2, MLP trained on direct-sum rows of 200 seeded
+           sims of the same size) against the oracle's FP64 softened direct sum on 1,024 sampled
+           targets: global relative error <= 1e-3 and per-body p99 <= P99_C4; then 10 checked
+           cycles with no fallback.
+config 5   two Plummer spheres of 524,288 bodies each (masses 1 : 3, 10 a apart, approaching at
+           0.5), elastic collisions, radii 0.05 m^(1/3), surrogate on a 64^3 grid with shell 3,
+           10 checked cycles: sampled force error against the oracle <= 1e-3, each cycle's pair
+           list identical to the oracle's detection (cell lists, pinned to the brute-force
+           detector) on the state the cycle started from, and FP64 energy and momentum drift
+           within the bounds below.
+Reference semantics followed: the softened force of cmd/body/octree.go:314-325, detection
+body.go:192-208 with the dispatch of :171-186, coll3d collisioncalc.go:42-186, Update
+body.go:114-139.
+"""
+import numpy as np
+import pytest
+
+from nbodygo_amd import NBodyGPU, gen
+
+pytestmark = pytest.mark.gpu
+
+NT = 16
+P99_C4 = 3e-3          # per-body 99th percentile of |da| / |a|, config 4
+E_DRIFT_C5 = 1e-6      # |E_10 - E_0| / |E_0| over 10 cycles, config 5
+P_DRIFT_C5 = 5e-6      # |P_10 - P_0| / sum m |v| over 10 cycles, config 5
+
+
+def errors(a, ref):
+    per = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return float(np.linalg.norm(a - ref) / np.linalg.norm(ref)), float(np.percentile(per, 99))
+
+
+def train(g, make, sims, rows, seed0, **sur):
+    for k in range(sims):
+        g.load(make(seed0 + k))
+        g.sur_enable(check_every=0, **sur)
+        g.sur_collect(rows, seed=k)
+    return g.sur_fit(steps=4000, batch=512, lr=1e-3)
+
+
+def test_config4_surrogate_at_size(oracle):
+    n = 262144
+    b = gen.plummer(n, seed=2024)
+    st = oracle.State.from_bodies(b)
+    idx = np.sort(np.random.default_rng(4).choice(n, 1024, replace=False))
+    ref = oracle.force_soft(st, 1.0, 0.01, idx=idx, nthreads=NT) / b.m[idx, None]
+    with NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=False) as g:
+        print("config 4: training on 200 sims", flush=True)
+        train(g, lambda s: gen.plummer(n, seed=s), 200, 2048, 100000)
+        g.load(b)
+        g.sur_enable(check_every=0)
+        a = g.forces()[idx] / b.m[idx, None]
+        g.sur_enable(check_every=1, check_samples=1024)
+        s0 = g.sur_stats()
+        g.step(1e-3, 1.0, nsteps=10)
+        s1 = g.sur_stats()
+    glob, p99 = errors(a, ref)
+    print("config 4: global %.3e p99 %.3e, %d checks, %d fallbacks"
+          % (glob, p99, s1["checks"] - s0["checks"], s1["fallbacks"] - s0["fallbacks"]),
+          flush=True)
+    assert glob <= 1e-3
+    assert p99 <= P99_C4
+    assert s1["checks"] - s0["checks"] == 10
+    assert s1["fallbacks"] == s0["fallbacks"]
+
+
+def test_config5_colliding_spheres_at_size(oracle):
+    n_each = 524288
+    b = gen.two_plummer(n_each, seed=2024, radius_k=0.05)
+    st0 = oracle.State.from_bodies(b)
+    idx = np.sort(np.random.default_rng(5).choice(b.n, 512, replace=False))
+    ref = oracle.force_soft(st0, 1.0, 0.01, idx=idx, nthreads=NT) / b.m[idx, None]
+    pscale = float(np.sum(b.m * np.linalg.norm(b.vel(), axis=1)))
+    sur = dict(grid=64, shell=3)
+    with NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=True) as g:
+        print("config 5: training on 8 sims", flush=True)
+        train(g, lambda s: gen.two_plummer(n_each, seed=s, radius_k=0.05), 8, 4096, 300000,
+              **sur)
+        g.load(b)
+        g.sur_enable(check_every=0, **sur)
+        a = g.forces()[idx] / b.m[idx, None]
+        g.load(b)
+        g.sur_enable(check_every=1, tol=1e-3, **sur)
+        d0 = g.diagnostics()
+        npairs = 0
+        for cycle in range(10):
+            cur = g.store()
+            s = g.step(1e-3, 1.0)
+            pairs = g.pairs()
+            exp = oracle.detect_grid(oracle.State.from_bodies(cur), nthreads=NT)
+            np.testing.assert_array_equal(pairs, exp, err_msg="cycle %d" % cycle)
+            npairs += len(pairs)
+        d1 = g.diagnostics()
+        stats = g.sur_stats()
+    glob, p99 = errors(a, ref)
+    e0, e1 = d0["kinetic"] + d0["potential"], d1["kinetic"] + d1["potential"]
+    e_drift = abs(e1 - e0) / abs(e0)
+    p_drift = float(np.linalg.norm(np.asarray(d1["momentum"]) - d0["momentum"]) / pscale)
+    print("config 5: force error %.3e (p99 %.3e), %d pairs in 10 cycles, energy drift %.2e, "
+          "momentum drift %.2e, %d fallbacks" % (glob, p99, npairs, e_drift, p_drift,
+                                                 stats["fallbacks"]), flush=True)
+    assert glob <= 1e-3
+    assert npairs > 0
+    assert e_drift <= E_DRIFT_C5
+    assert p_drift <= P_DRIFT_C5

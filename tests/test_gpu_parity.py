@@ -96,6 +96,38 @@ def test_fp64_steps_with_collisions_track_the_oracle(oracle):
     assert total_pairs > 100
 
 
+# Config 1's softened variant: softening "3e4 * scale" read with scale = the runner's time scaling
+# (dt = 1e-3, the -m flag, main.go:199), so eps = 30 length units against a mean spacing of about
+# 100 in the radius-1e3 sphere; radii 0.05 m^(1/3) as in reference mode.
+C1_EPS = 3e4 * 1e-3
+C1_RADIUS_K = 0.05
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_config1_softened_tracks_the_oracle(oracle, precision):
+    """Config 1 with eps = 30, overlap gate and elastic collisions, 100 cycles, each GPU cycle
+    against one oracle cycle from the same state: identical pair sets every cycle; FP64 states
+    equal up to coll3d's transcendental ulps, FP32 states within the FP32 force error."""
+    b = c1(radius_k=C1_RADIUS_K)
+    rtol_v = 1e-12 if precision == "fp64" else 1e-6
+    with NBodyGPU(eps=C1_EPS, precision=precision) as g:
+        g.load(b)
+        total_pairs = 0
+        for step in range(100):
+            cur = g.store()
+            st = oracle.State.from_bodies(cur)
+            g.step(1e-3, 1.0)
+            o = oracle.step(st, G_REF, C1_EPS, True, True, 1e-3, 1.0, nthreads=NT)
+            np.testing.assert_array_equal(g.pairs(), o["pairs"])
+            total_pairs += len(o["pairs"])
+            nxt = g.store()
+            np.testing.assert_allclose(nxt.pos(), st.pos(), rtol=1e-12,
+                                       atol=1e-12 * np.abs(st.pos()).max())
+            np.testing.assert_allclose(nxt.vel(), st.vel(), rtol=rtol_v,
+                                       atol=rtol_v * np.abs(st.vel()).max())
+    assert total_pairs > 100
+
+
 # ---------------------------------------------------------------------------------------------
 # FP32 path: error against FP64, exact pair sets
 # ---------------------------------------------------------------------------------------------
