@@ -266,10 +266,12 @@ typedef struct nb_bh_params {
   int32_t group;     /* bodies per wavefront in the tree walk: 16, 32 or 64; 0 = 64 */
   int32_t mode;      /* 0 = exact: each body walks the tree with the reference's test and FP64
                         terms in the reference's order; 1 = fast: the 64 bodies of a wavefront
-                        (neighbours in key order) share one walk that opens every node any of
-                        them would open (theta against the distance to the group's bounding box),
-                        and the terms are summed in FP32 (the body's own term excluded); forces
-                        then differ from exact mode by the FP32 and opening differences */
+                        (neighbours in key order), cut at tree-cell boundaries into groups that
+                        each lie in a cell of fewer than 1,088 bodies, share one walk per group
+                        that opens every node any of them would open (theta against the distance
+                        to the group's bounding box), and the terms are summed in FP32 (the
+                        body's own term excluded); forces then differ from exact mode by the
+                        FP32 and opening differences */
   int32_t reserved[5];
 } nb_bh_params;
 

@@ -98,7 +98,8 @@ __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, d
     for (int k = 0; k < IPL; k++) px[k] = py[k] = pz[k] = 0.f;
     for (int q = lane; q < cnt; q += 64) tl[q] = js[it.s0 + t0 + q];
     wave_lds_sync();
-#pragma unroll 4
+    // cnt is a multiple of 64: sixteen LDS reads per iteration share one base address
+#pragma unroll 16
     for (int jj = 0; jj < cnt; jj++) {
       const float4 b = tl[jj];
       float dx[IPL], dy[IPL], dz[IPL], inv[IPL];

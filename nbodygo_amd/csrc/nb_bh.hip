@@ -477,6 +477,7 @@ struct BhForceArgs {
   const BhCounters *cnt;
   BhCounters *cnt_w;
   const int32_t *perm;
+  const uint64_t *keys;  // sorted keys (fast mode's group cut)
   const uint8_t *flags;
   const double *sx, *sy, *sz, *sm;
   const double4 *nbox, *nmc;
@@ -587,10 +588,13 @@ __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
 }
 
 // ---- fast mode: one walk per group of bodies, FP32 terms ------------------------------------------
-// The 64 bodies of a wave (consecutive in key order, so close in space) share one traversal.  A
-// wave whose bodies straddle separate regions of key order (its box more than 8 times the summed
-// volume of its four 16-body quarters' boxes) walks the tree once per quarter instead.  A
-// node is accepted for the whole group when 2 half < theta * sqrt(d_box^2 + soft^2), d_box being
+// The 64 bodies of a wave (consecutive in key order) are cut into groups at tree-cell boundaries:
+// a run whose smallest enclosing cell holds kBhGroupSpan or more bodies on either side of the run
+// is split between that cell's children, and again, until every run lies in a cell of fewer than
+// 64 + 2 kBhGroupSpan bodies (or is one body, or one bucket).  Each group shares one traversal.
+// Without the cut, a run that straddles two large cells takes a box spanning both and opens
+// everything inside it to the leaves (on a Plummer sphere one such wave met 261,464 of the
+// 262,144 bodies and set the kernel's duration).  A node is accepted for the whole group when 2 half < theta * sqrt(d_box^2 + soft^2), d_box being
 // its centre of mass's distance to the group's bounding box: every body of the group is then at
 // least that far away, so each would accept it by the reference's own test (octree.go:299-309);
 // otherwise the node is opened.  The walk is breadth-parallel: each step pops up to 8 nodes from
@@ -599,6 +603,7 @@ __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
 // evaluated by all lanes against their own bodies like the direct FP32 kernel (sources broadcast
 // from LDS, coordinates relative to the group's centre, r^-3 from v_rsq_f32), the list's partial
 // sums are added into FP64 per lane, and the self term is skipped by body index.
+constexpr int kBhGroupSpan = 512;   // see the group cut above
 constexpr int kBhFastStack = 2048;  // per-wave stack entries (bound: 64 per level, 21 levels)
 constexpr int kBhFastList = 128;    // per-wave term list (< 64 kept + up to 64 added per step)
 
@@ -690,45 +695,55 @@ __global__ __launch_bounds__(256) void k_bh_force_fast(BhForceArgs a) {
   }
   if (__builtin_amdgcn_ballot_w64(act) == 0) return;
   const double px = act ? a.sx[s] : 0, py = act ? a.sy[s] : 0, pz = act ? a.sz[s] : 0;
-  // boxes of the 16-lane quarters and of the whole wave (FP64)
-  double lo[3] = {act ? px : DBL_MAX, act ? py : DBL_MAX, act ? pz : DBL_MAX};
-  double hi[3] = {act ? px : -DBL_MAX, act ? py : -DBL_MAX, act ? pz : -DBL_MAX};
-  for (int o = 8; o > 0; o >>= 1)
-    for (int d = 0; d < 3; d++) {
-      lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
-      hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+  // the group cut (wave-uniform): bit j of `starts` marks a group starting at lane j
+  const int wbase = s - lane, nl = min(64, M - wbase);
+  const uint64_t kl = lane < nl ? a.keys[s] : ~0ull;
+  const uint64_t kprev = __shfl_up(kl, 1);
+  unsigned long long starts = 1ull, fin = 0ull;
+  for (;;) {
+    const unsigned long long todo = starts & ~fin;
+    if (todo == 0) break;
+    const int ga = __builtin_ctzll(todo);
+    const unsigned long long later = starts & ~((2ull << ga) - 1ull);
+    const int gb = later ? __builtin_ctzll(later) : nl;
+    const uint64_t ka = __shfl(kl, ga), kb = __shfl(kl, gb - 1);
+    const uint64_t x = ka ^ kb;  // live keys use bits 0..62, so x < 2^63
+    bool cut = false;
+    if (gb - ga > 1 && x != 0) {
+      // bits below the run's common prefix: whole octal digits down from the highest difference
+      const int lb = 3 * ((63 - __builtin_clzll(x)) / 3 + 1);
+      const uint64_t pre = ka >> lb;
+      const int pa = wbase + ga - kBhGroupSpan, pb = wbase + gb - 1 + kBhGroupSpan;
+      const bool big = (pa >= 0 && (a.keys[pa] >> lb) == pre) ||
+                       (pb < M && (a.keys[pb] >> lb) == pre);
+      if (big) {
+        // cut where the next digit changes: the run's parts in the cell's children
+        starts |= __builtin_amdgcn_ballot_w64(lane > ga && lane < gb &&
+                                              ((kl ^ kprev) >> (lb - 3)) != 0);
+        cut = true;
+      }
     }
-  double qlo[3] = {lo[0], lo[1], lo[2]}, qhi[3] = {hi[0], hi[1], hi[2]};  // this quarter's box
-  auto vol = [](const double *l, const double *h) {
-    return l[0] <= h[0] ? (h[0] - l[0]) * (h[1] - l[1]) * (h[2] - l[2]) : 0.0;
-  };
-  double qv = vol(qlo, qhi);
-  for (int o = 32; o >= 16; o >>= 1) {
-    for (int d = 0; d < 3; d++) {
-      lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
-      hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
-    }
+    if (!cut) fin |= 1ull << ga;
   }
-  double sumq = qv;  // sum of the four quarter volumes (each quarter's lanes hold its own)
-  sumq += __shfl_xor(sumq, 16);
-  sumq += __shfl_xor(sumq, 32);
-  // a wave whose bodies straddle separate regions (its box far larger than its quarters' boxes)
-  // walks the tree once per quarter, with the quarter's box; otherwise once for all 64 bodies
-  const bool split = vol(lo, hi) > 8.0 * sumq + 1e-300;
-  const int ngroups = split ? 4 : 1;
   const double th2 = a.theta * a.theta, soft2 = a.soft2;
   double ax = 0, ay = 0, az = 0;
   unsigned long long nint = 0;
   bool overflow = false;
   const int node_lim = (int)min(a.node_cap, (int64_t)INT32_MAX / 8);
-  for (int grp = 0; grp < ngroups && !overflow; grp++) {
-    double glo[3], ghi[3];
-    for (int d = 0; d < 3; d++) {
-      glo[d] = split ? __shfl(qlo[d], grp * 16) : lo[d];
-      ghi[d] = split ? __shfl(qhi[d], grp * 16) : hi[d];
-    }
-    const bool mine = act && (!split || (lane >> 4) == grp);
+  for (unsigned long long segs = starts; segs && !overflow;) {
+    const int ga = __builtin_ctzll(segs);
+    segs &= segs - 1;
+    const int gb = segs ? __builtin_ctzll(segs) : nl;
+    const bool mine = act && lane >= ga && lane < gb;
     if (__builtin_amdgcn_ballot_w64(mine) == 0) continue;
+    // the group's box (FP64)
+    double glo[3] = {mine ? px : DBL_MAX, mine ? py : DBL_MAX, mine ? pz : DBL_MAX};
+    double ghi[3] = {mine ? px : -DBL_MAX, mine ? py : -DBL_MAX, mine ? pz : -DBL_MAX};
+    for (int o = 32; o > 0; o >>= 1)
+      for (int d = 0; d < 3; d++) {
+        glo[d] = fmin(glo[d], __shfl_xor(glo[d], o));
+        ghi[d] = fmax(ghi[d], __shfl_xor(ghi[d], o));
+      }
     const double cx = 0.5 * (glo[0] + ghi[0]), cy = 0.5 * (glo[1] + ghi[1]),
                  cz = 0.5 * (glo[2] + ghi[2]);
     const double hx = 0.5 * (ghi[0] - glo[0]), hy = 0.5 * (ghi[1] - glo[1]),
@@ -1136,6 +1151,7 @@ int bh_force_phase(Ctx *c, bool timed, int ev_slot) {
   fa.cnt = b->cnt;
   fa.cnt_w = b->cnt;
   fa.perm = b->perm;
+  fa.keys = b->keys;
   fa.flags = c->flags;
   fa.sx = b->sx; fa.sy = b->sy; fa.sz = b->sz; fa.sm = b->sm;
   fa.nbox = b->nbox;

@@ -237,10 +237,14 @@ __global__ __launch_bounds__(256) void k_force_lds(ForceArgs a) {
     float px[IPL], py[IPL], pz[IPL], rmin[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; k++) px[k] = py[k] = pz[k] = 0.f, rmin[k] = FLT_MAX;
-#pragma unroll 4
-    for (int jj = 0; jj < a.tile_j; jj++) {
-      const float4 b = sh[jj];
-      NB_INTERACT(b.x, b.y, b.z, b.w)
+    // tile_j is a multiple of 64.  Eight sources per step are read first (eight ds_read_b128 in
+    // flight, one LDS base address with immediate offsets), then used in order.
+    for (int jj = 0; jj < a.tile_j; jj += 8) {
+      float4 b8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) b8[u] = sh[jj + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) NB_INTERACT(b8[u].x, b8[u].y, b8[u].z, b8[u].w)
     }
     self_tile_rmin<IPL, DETECT>(T, rmin, base, stride, t, a.tile_j, eps2, [&](int jj) -> f4v {
       const float4 b = sh[jj];

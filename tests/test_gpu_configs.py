@@ -3,8 +3,8 @@
 config 4   Plummer sphere of N = 262,144 (a = 1, M = 1, G = 1, eps = 0.01): the learned far-field
            surrogate (32^3 stretched grid, shell 2, MLP trained on direct-sum rows of 200 seeded
            sims of the same size) against the oracle's FP64 softened direct sum on 1,024 sampled
-           targets: global relative error <= 1e-3 and per-body p99 <= P99_C4; then 10 checked
-           cycles with no fallback.
+           targets: global relative error <= 1e-3 and per-body p99 <= P99_C4 (2e-3); then 10
+           checked cycles with no fallback.
 config 5   two Plummer spheres of 524,288 bodies each (masses 1 : 3, 10 a apart, approaching at
            0.5), elastic collisions, radii 0.05 m^(1/3), surrogate on a 64^3 grid with shell 3,
            10 checked cycles: sampled force error against the oracle <= 1e-3, each cycle's pair
@@ -23,7 +23,7 @@ from nbodygo_amd import NBodyGPU, gen
 pytestmark = pytest.mark.gpu
 
 NT = 16
-P99_C4 = 3e-3          # per-body 99th percentile of |da| / |a|, config 4
+P99_C4 = 2e-3          # per-body 99th percentile of |da| / |a|, config 4
 E_DRIFT_C5 = 1e-6      # |E_10 - E_0| / |E_0| over 10 cycles, config 5
 P_DRIFT_C5 = 5e-6      # |P_10 - P_0| / sum m |v| over 10 cycles, config 5
 
