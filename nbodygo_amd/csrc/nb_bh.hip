@@ -604,7 +604,10 @@ __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
 // from LDS, coordinates relative to the group's centre, r^-3 from v_rsq_f32), the list's partial
 // sums are added into FP64 per lane, and the self term is skipped by body index.
 constexpr int kBhGroupSpan = 512;   // see the group cut above
-constexpr int kBhFastStack = 2048;  // per-wave stack entries (bound: 64 per level, 21 levels)
+// Per-wave stack entries.  The stack stays sorted by depth (each step pops the deepest entries
+// and pushes their children on top), so entries of depth d can only be pushed by a step that
+// popped every deeper-or-equal entry: at most 64 entries per depth, 21 depths below the root.
+constexpr int kBhFastStack = 64 * kBhLevels;
 constexpr int kBhFastList = 128;    // per-wave term list (< 64 kept + up to 64 added per step)
 
 struct BhFastLds {
