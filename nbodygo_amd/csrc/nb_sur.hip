@@ -44,6 +44,11 @@ constexpr int kFarUsed = 34;  // words of the record the far-field kernel writes
 constexpr int kFeat = 30;     // MLP input width
 constexpr int kMaxGrid = 64;  // cell keys 0..64^3 (the last for inactive bodies): 19 bits
 // Bits of the cell keys 0..ncell sorted by the radix sort (16 up to 40^3 cells).
+// Stable radix sort by onesweep passes at every size (rocPRIM's default sorts inputs below a
+// million items by block sort plus merge passes: 17 launches for 262,144 keys).
+using SortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                           rocprim::default_config, 0>;
+
 static unsigned key_bits(int ncell) { return 32u - (unsigned)__builtin_clz((unsigned)ncell); }
 
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -673,7 +678,14 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   const int t_begin = c_begin + it.y * 64, t_end = min(c_end, t_begin + 64);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int k = t_begin + lane;
+  // A wave holds `copies` copies of the item's targets (the most, up to 8, that fit in 64
+  // lanes); copy cp takes every copies-th source of each tile and of the shell, and the copies'
+  // partial sums are combined in a fixed order at the end.
+  const int tcount = t_end - t_begin;
+  const int copies = tcount > 32 ? 1 : tcount > 16 ? 2 : tcount > 8 ? 4 : 8;
+  const int cwid = 64 / copies;  // lanes per copy
+  const int cp = lane / cwid;
+  const int k = t_begin + lane % cwid;
   const bool tv = k < t_end;
   const float4 me = a.sp[tv ? k : t_begin];
   const float eps2 = to_vgpr(a.eps2);
@@ -732,29 +744,36 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
         }
         wave_lds_sync();
         float px = 0, py = 0, pz = 0, r2min = FLT_MAX;
-        const int cnt4 = (cnt + 3) & ~3;
+        // sources j = cp, cp + copies, ...: four per step, padded with zero-mass entries (the
+        // padded count is a multiple of 4 copies and at most 64, the tile's size)
+        const int q4 = 4 * copies;
+        const int cntp = (cnt + q4 - 1) / q4 * q4;
         if (COLL) {
           // the body's own entry (r^2 = eps^2) must not count as a candidate
           if (s0 < t_end && s0 + cnt > t_begin) {
             const int self = k - s0;
-            for (int j = 0; j < cnt; j++) {
+            for (int j = cp; j < cnt; j += copies) {
               const float keep = r2min;
               near_pair_min(tl[j], me, eps2, px, py, pz, r2min);
               if (j == self) r2min = keep;
             }
           } else {
             // zero-mass padding may only add candidates (an exact pass over the real sources)
-            for (int j = 0; j < cnt4; j += 4) {
-              const float4 b0 = tl[j], b1 = tl[j + 1], b2 = tl[j + 2], b3 = tl[j + 3];
+            for (int j = cp; j < cntp; j += q4) {
+              const float4 b0 = tl[j], b1 = tl[j + copies], b2 = tl[j + 2 * copies],
+                           b3 = tl[j + 3 * copies];
               near_pair_min(b0, me, eps2, px, py, pz, r2min);
               near_pair_min(b1, me, eps2, px, py, pz, r2min);
               near_pair_min(b2, me, eps2, px, py, pz, r2min);
               near_pair_min(b3, me, eps2, px, py, pz, r2min);
             }
           }
+          // the chunk's minimum over all copies, for the target's one decision below
+          for (int o = cwid; o < 64; o <<= 1) r2min = fminf(r2min, __shfl_xor(r2min, o));
         } else {
-          for (int j = 0; j < cnt4; j += 4) {
-            const float4 b0 = tl[j], b1 = tl[j + 1], b2 = tl[j + 2], b3 = tl[j + 3];
+          for (int j = cp; j < cntp; j += q4) {
+            const float4 b0 = tl[j], b1 = tl[j + copies], b2 = tl[j + 2 * copies],
+                         b3 = tl[j + 3 * copies];
             near_pair(b0, me, eps2, px, py, pz);
             near_pair(b1, me, eps2, px, py, pz);
             near_pair(b2, me, eps2, px, py, pz);
@@ -762,7 +781,7 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
           }
         }
         wave_lds_sync();
-        if (COLL && tv) {
+        if (COLL && tv && cp == 0) {
           // |d_fp32 - d| <= e for every pair of this chunk; 2^-20 covers the rounding of both
           // positions to FP32 and of their differences with room to spare
           const float e = 9.5367431640625e-7f * (me_inf + xmax_c + ri + rmax_c);
@@ -814,7 +833,7 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
       }
       wave_lds_sync();
       const int nsel = __builtin_popcountll(mm);
-      for (int j = 0; j < nsel; j++)
+      for (int j = cp; j < nsel; j += copies)
         quad_field(rl[j][0], rl[j][1], rl[j][2], me.x, me.y, me.z, eps2, sx, sy, sz);
       wave_lds_sync();
       q += __builtin_popcountll(okm);
@@ -831,10 +850,15 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   __syncthreads();
   if (wave != 0) return;
   if (lane == 0) atomicAdd(a.near_pairs, (double)(t_end - t_begin) * (double)nsrc);
-  if (!tv) return;
   ax = ((ax + red[0][0][lane]) + red[1][0][lane]) + red[2][0][lane];
   ay = ((ay + red[0][1][lane]) + red[1][1][lane]) + red[2][1][lane];
   az = ((az + red[0][2][lane]) + red[1][2][lane]) + red[2][2][lane];
+  for (int o = cwid; o < 64; o <<= 1) {  // the copies, pairwise
+    ax += __shfl_xor(ax, o);
+    ay += __shfl_xor(ay, o);
+    az += __shfl_xor(az, o);
+  }
+  if (!tv || cp != 0) return;
   // third-order Taylor step of the far expansion from the cell's centre of mass
   cfp cr = (cfp)(a.crec + (size_t)c * kCellRec);
   cfp fe = (cfp)(a.fexp + (size_t)c * kFarRec);
@@ -1187,9 +1211,9 @@ static int sur_ensure_bodies(Ctx *c) {
   NB_HIP(c, salloc(&s->ymlp, n * 3));
   NB_HIP(c, salloc(&s->feat, n * kFeat));
   size_t bytes = 0;
-  NB_HIP(c, rocprim::radix_sort_pairs(nullptr, bytes, s->keys, s->keys2, s->perm_in, s->perm,
-                                      (size_t)n, 0u, key_bits(kMaxGrid * kMaxGrid * kMaxGrid),
-                                      c->stream));
+  NB_HIP(c, rocprim::radix_sort_pairs<SortCfg>(
+                nullptr, bytes, s->keys, s->keys2, s->perm_in, s->perm, (size_t)n, 0u,
+                key_bits(kMaxGrid * kMaxGrid * kMaxGrid), c->stream));
   void *tmp = nullptr;
   NB_HIP(c, hipMalloc(&tmp, bytes ? bytes : 1));
   if (s->sort_tmp) (void)hipFree(s->sort_tmp);
@@ -1257,8 +1281,8 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
                      n, s->center, s->r0, G, s->keys, s->perm_in);
   NB_HIP(c, hipGetLastError());
   size_t bytes = s->sort_bytes;
-  NB_HIP(c, rocprim::radix_sort_pairs(s->sort_tmp, bytes, s->keys, s->keys2, s->perm_in, s->perm,
-                                      (size_t)n, 0u, key_bits(ncell), st));
+  NB_HIP(c, rocprim::radix_sort_pairs<SortCfg>(s->sort_tmp, bytes, s->keys, s->keys2, s->perm_in,
+                                               s->perm, (size_t)n, 0u, key_bits(ncell), st));
   hipLaunchKernelGGL(k_sur_bounds, dim3((unsigned)(ncell / 1024 + 1)), dim3(1024), 0, st,
                      s->keys2, n, ncell, s->cstart, s->btot);
   hipLaunchKernelGGL(k_sur_scan, dim3((unsigned)((ncell + 1023) / 1024)), dim3(1024), 0, st,
