@@ -426,9 +426,25 @@ __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int
     const bool valid = t < nec;
     const float *tr = crec_ne + (size_t)(valid ? t : nec - 1) * kCellRec;
     const float X = tr[0], Y = tr[1], Z = tr[2], ix = tr[11], iy = tr[12], iz = tr[13];
-    float o[kFarUsed];
+    // Per source the monopole's derivative tensors are
+    //   D1_a = -m3 r_a,  D2_ab = 3 m5 r_a r_b - d_ab m3,
+    //   D3_abc = 3 m5 (d_ab r_c + d_ac r_b + d_bc r_a) - 15 m7 r_a r_b r_c,
+    //   D4_abcd = 105 m9 r_a r_b r_c r_d - 15 m7 (six d r r) + 3 m5 (three d d),
+    // with m_k = M / |r|^k.  They are linear in the sums below, so the loop accumulates those
+    // and the Kronecker terms are formed once per target after it.  Index orders: pairs
+    // (xx yy zz xy xz yz), triples (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz), quadruples
+    // (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz xzzz yyyy yyyz yyzz yzzz zzzz).
+    float qf[3] = {0.f, 0.f, 0.f};   // quadrupole field
+    float a1[3] = {0.f, 0.f, 0.f};   // sum m3 r
+    float a0 = 0.f, b0 = 0.f;        // sum m3, sum m5
+    float b1[3] = {0.f, 0.f, 0.f};   // sum m5 r
+    float b2[6], c2[6], c3[10], e4[15];  // sum m5 rr, m7 rr, m7 rrr, m9 rrrr
 #pragma unroll
-    for (int q = 0; q < kFarUsed; q++) o[q] = 0.f;
+    for (int q = 0; q < 6; q++) b2[q] = c2[q] = 0.f;
+#pragma unroll
+    for (int q = 0; q < 10; q++) c3[q] = 0.f;
+#pragma unroll
+    for (int q = 0; q < 15; q++) e4[q] = 0.f;
     const int per = ms.width / 4;
     const int s0 = sl * ms.width + wave * per, s1 = min(s0 + per, nec);
     for (int sb = s0; sb < s1; sb += 64) {
@@ -450,64 +466,110 @@ __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int
         float inv = __builtin_amdgcn_rsqf(rr);
         inv = cheb > Kf ? inv : 0.f;
         const float inv2 = inv * inv, M = r0.w;
-        const float m3 = M * inv2 * inv, m5 = m3 * inv2, m7 = m5 * inv2;
-        o[0] -= m3 * rx;
-        o[1] -= m3 * ry;
-        o[2] -= m3 * rz;
+        const float m3 = M * inv2 * inv, m5 = m3 * inv2, m7 = m5 * inv2, m9 = m7 * inv2;
         // quadrupole field
-        const float qx = r1.x * rx + r1.w * ry + r2v.x * rz;
-        const float qy = r1.w * rx + r1.y * ry + r2v.y * rz;
-        const float qz = r2v.x * rx + r2v.y * ry + r1.z * rz;
-        const float rqr = qx * rx + qy * ry + qz * rz;
+        // (the accumulations are written as fused multiply-adds: the build keeps a * b + c
+        // unfused unless asked)
+        const float qx = __builtin_fmaf(r2v.x, rz, __builtin_fmaf(r1.w, ry, r1.x * rx));
+        const float qy = __builtin_fmaf(r2v.y, rz, __builtin_fmaf(r1.y, ry, r1.w * rx));
+        const float qz = __builtin_fmaf(r1.z, rz, __builtin_fmaf(r2v.y, ry, r2v.x * rx));
+        const float rqr = __builtin_fmaf(qz, rz, __builtin_fmaf(qy, ry, qx * rx));
         const float i5 = inv2 * inv2 * inv, i7 = i5 * inv2;
         const float c7 = 2.5f * rqr * i7;
-        o[0] += qx * i5 - c7 * rx;
-        o[1] += qy * i5 - c7 * ry;
-        o[2] += qz * i5 - c7 * rz;
-        // Jacobian of the monopole field (xx yy zz xy xz yz)
-        const float t3 = 3.f * m5;
-        o[3] += t3 * rx * rx - m3;
-        o[4] += t3 * ry * ry - m3;
-        o[5] += t3 * rz * rz - m3;
-        o[6] += t3 * rx * ry;
-        o[7] += t3 * rx * rz;
-        o[8] += t3 * ry * rz;
-        // Hessian of the monopole field (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz)
-        const float f5 = 15.f * m7;
-        const float ax5 = t3 * rx, ay5 = t3 * ry, az5 = t3 * rz;
-        o[9] += 3.f * ax5 - f5 * rx * rx * rx;
-        o[10] += ay5 - f5 * rx * rx * ry;
-        o[11] += az5 - f5 * rx * rx * rz;
-        o[12] += ax5 - f5 * rx * ry * ry;
-        o[13] += -f5 * rx * ry * rz;
-        o[14] += ax5 - f5 * rx * rz * rz;
-        o[15] += 3.f * ay5 - f5 * ry * ry * ry;
-        o[16] += az5 - f5 * ry * ry * rz;
-        o[17] += ay5 - f5 * ry * rz * rz;
-        o[18] += 3.f * az5 - f5 * rz * rz * rz;
-        // third derivatives of the monopole field (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz
-        // xzzz yyyy yyyz yyzz yzzz zzzz): 3 m5 (dd + dd + dd) - 15 m7 (six d r r) + 105 m9 rrrr
-        const float g9 = 105.f * m7 * inv2, xx = rx * rx, yy = ry * ry, zz = rz * rz;
+        qf[0] = __builtin_fmaf(qx, i5, __builtin_fmaf(-c7, rx, qf[0]));
+        qf[1] = __builtin_fmaf(qy, i5, __builtin_fmaf(-c7, ry, qf[1]));
+        qf[2] = __builtin_fmaf(qz, i5, __builtin_fmaf(-c7, rz, qf[2]));
+        // monopole sums
+        a1[0] = __builtin_fmaf(m3, rx, a1[0]);
+        a1[1] = __builtin_fmaf(m3, ry, a1[1]);
+        a1[2] = __builtin_fmaf(m3, rz, a1[2]);
+        a0 += m3;
+        b0 += m5;
+        const float m5x = m5 * rx, m5y = m5 * ry, m5z = m5 * rz;
+        b1[0] += m5x;
+        b1[1] += m5y;
+        b1[2] += m5z;
+        b2[0] = __builtin_fmaf(m5x, rx, b2[0]);
+        b2[1] = __builtin_fmaf(m5y, ry, b2[1]);
+        b2[2] = __builtin_fmaf(m5z, rz, b2[2]);
+        b2[3] = __builtin_fmaf(m5x, ry, b2[3]);
+        b2[4] = __builtin_fmaf(m5x, rz, b2[4]);
+        b2[5] = __builtin_fmaf(m5y, rz, b2[5]);
+        const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
         const float xy = rx * ry, xz = rx * rz, yz = ry * rz;
-        const float u45 = 3.f * f5, u9 = 3.f * t3;  // 45 m7, 9 m5
-        o[19] += u9 - 6.f * f5 * xx + g9 * xx * xx;
-        o[20] += -u45 * xy + g9 * xx * xy;
-        o[21] += -u45 * xz + g9 * xx * xz;
-        o[22] += t3 - f5 * (xx + yy) + g9 * xx * yy;
-        o[23] += -f5 * yz + g9 * xx * yz;
-        o[24] += t3 - f5 * (xx + zz) + g9 * xx * zz;
-        o[25] += -u45 * xy + g9 * xy * yy;
-        o[26] += -f5 * xz + g9 * xz * yy;
-        o[27] += -f5 * xy + g9 * xy * zz;
-        o[28] += -u45 * xz + g9 * xz * zz;
-        o[29] += u9 - 6.f * f5 * yy + g9 * yy * yy;
-        o[30] += -u45 * yz + g9 * yy * yz;
-        o[31] += t3 - f5 * (yy + zz) + g9 * yy * zz;
-        o[32] += -u45 * yz + g9 * yz * zz;
-        o[33] += u9 - 6.f * f5 * zz + g9 * zz * zz;
+        const float cxx = m7 * xx, cyy = m7 * yy, czz = m7 * zz;
+        const float cxy = m7 * xy, cxz = m7 * xz, cyz = m7 * yz;
+        c2[0] += cxx;
+        c2[1] += cyy;
+        c2[2] += czz;
+        c2[3] += cxy;
+        c2[4] += cxz;
+        c2[5] += cyz;
+        c3[0] = __builtin_fmaf(cxx, rx, c3[0]);
+        c3[1] = __builtin_fmaf(cxx, ry, c3[1]);
+        c3[2] = __builtin_fmaf(cxx, rz, c3[2]);
+        c3[3] = __builtin_fmaf(cyy, rx, c3[3]);
+        c3[4] = __builtin_fmaf(cxy, rz, c3[4]);
+        c3[5] = __builtin_fmaf(czz, rx, c3[5]);
+        c3[6] = __builtin_fmaf(cyy, ry, c3[6]);
+        c3[7] = __builtin_fmaf(cyy, rz, c3[7]);
+        c3[8] = __builtin_fmaf(czz, ry, c3[8]);
+        c3[9] = __builtin_fmaf(czz, rz, c3[9]);
+        const float exx = m9 * xx, eyy = m9 * yy, ezz = m9 * zz;
+        e4[0] = __builtin_fmaf(exx, xx, e4[0]);
+        e4[1] = __builtin_fmaf(exx, xy, e4[1]);
+        e4[2] = __builtin_fmaf(exx, xz, e4[2]);
+        e4[3] = __builtin_fmaf(exx, yy, e4[3]);
+        e4[4] = __builtin_fmaf(exx, yz, e4[4]);
+        e4[5] = __builtin_fmaf(exx, zz, e4[5]);
+        e4[6] = __builtin_fmaf(eyy, xy, e4[6]);
+        e4[7] = __builtin_fmaf(eyy, xz, e4[7]);
+        e4[8] = __builtin_fmaf(ezz, xy, e4[8]);
+        e4[9] = __builtin_fmaf(ezz, xz, e4[9]);
+        e4[10] = __builtin_fmaf(eyy, yy, e4[10]);
+        e4[11] = __builtin_fmaf(eyy, yz, e4[11]);
+        e4[12] = __builtin_fmaf(eyy, zz, e4[12]);
+        e4[13] = __builtin_fmaf(ezz, yz, e4[13]);
+        e4[14] = __builtin_fmaf(ezz, zz, e4[14]);
       }
       wave_lds_sync();
     }
+    // the tensors from the sums: field, Jacobian, Hessian, third derivatives
+    float o[kFarUsed];
+    o[0] = qf[0] - a1[0];
+    o[1] = qf[1] - a1[1];
+    o[2] = qf[2] - a1[2];
+    o[3] = 3.f * b2[0] - a0;
+    o[4] = 3.f * b2[1] - a0;
+    o[5] = 3.f * b2[2] - a0;
+    o[6] = 3.f * b2[3];
+    o[7] = 3.f * b2[4];
+    o[8] = 3.f * b2[5];
+    o[9] = 9.f * b1[0] - 15.f * c3[0];
+    o[10] = 3.f * b1[1] - 15.f * c3[1];
+    o[11] = 3.f * b1[2] - 15.f * c3[2];
+    o[12] = 3.f * b1[0] - 15.f * c3[3];
+    o[13] = -15.f * c3[4];
+    o[14] = 3.f * b1[0] - 15.f * c3[5];
+    o[15] = 9.f * b1[1] - 15.f * c3[6];
+    o[16] = 3.f * b1[2] - 15.f * c3[7];
+    o[17] = 3.f * b1[1] - 15.f * c3[8];
+    o[18] = 9.f * b1[2] - 15.f * c3[9];
+    o[19] = 105.f * e4[0] - 90.f * c2[0] + 9.f * b0;
+    o[20] = 105.f * e4[1] - 45.f * c2[3];
+    o[21] = 105.f * e4[2] - 45.f * c2[4];
+    o[22] = 105.f * e4[3] - 15.f * (c2[0] + c2[1]) + 3.f * b0;
+    o[23] = 105.f * e4[4] - 15.f * c2[5];
+    o[24] = 105.f * e4[5] - 15.f * (c2[0] + c2[2]) + 3.f * b0;
+    o[25] = 105.f * e4[6] - 45.f * c2[3];
+    o[26] = 105.f * e4[7] - 15.f * c2[4];
+    o[27] = 105.f * e4[8] - 15.f * c2[3];
+    o[28] = 105.f * e4[9] - 45.f * c2[4];
+    o[29] = 105.f * e4[10] - 90.f * c2[1] + 9.f * b0;
+    o[30] = 105.f * e4[11] - 45.f * c2[5];
+    o[31] = 105.f * e4[12] - 15.f * (c2[1] + c2[2]) + 3.f * b0;
+    o[32] = 105.f * e4[13] - 45.f * c2[5];
+    o[33] = 105.f * e4[14] - 90.f * c2[2] + 9.f * b0;
     if (wave > 0)
       for (int q = 0; q < kFarUsed; q++) red[wave - 1][q][lane] = o[q];
     __syncthreads();
@@ -544,14 +606,15 @@ __device__ __forceinline__ void quad_field(f4v r0, f4v r1, f4v r2v, float px, fl
   r2 = __builtin_fmaf(rz, rz, r2);
   const float inv = __builtin_amdgcn_rsqf(r2);
   const float inv2 = inv * inv, m3 = r0.w * inv2 * inv;
-  const float qx = r1.x * rx + r1.w * ry + r2v.x * rz;
-  const float qy = r1.w * rx + r1.y * ry + r2v.y * rz;
-  const float qz = r2v.x * rx + r2v.y * ry + r1.z * rz;
-  const float rqr = qx * rx + qy * ry + qz * rz;
+  const float qx = __builtin_fmaf(r2v.x, rz, __builtin_fmaf(r1.w, ry, r1.x * rx));
+  const float qy = __builtin_fmaf(r2v.y, rz, __builtin_fmaf(r1.y, ry, r1.w * rx));
+  const float qz = __builtin_fmaf(r1.z, rz, __builtin_fmaf(r2v.y, ry, r2v.x * rx));
+  const float rqr = __builtin_fmaf(qz, rz, __builtin_fmaf(qy, ry, qx * rx));
   const float i5 = inv2 * inv2 * inv, c7 = 2.5f * rqr * i5 * inv2;
-  ax += -m3 * rx + qx * i5 - c7 * rx;
-  ay += -m3 * ry + qy * i5 - c7 * ry;
-  az += -m3 * rz + qz * i5 - c7 * rz;
+  const float k = -(m3 + c7);  // monopole and quadrupole radial terms together
+  ax = __builtin_fmaf(qx, i5, __builtin_fmaf(k, rx, ax));
+  ay = __builtin_fmaf(qy, i5, __builtin_fmaf(k, ry, ay));
+  az = __builtin_fmaf(qz, i5, __builtin_fmaf(k, rz, az));
 }
 
 __device__ __forceinline__ void near_pair(f4v b, float4 me, float eps2, float &px, float &py,
