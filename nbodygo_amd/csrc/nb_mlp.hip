@@ -237,18 +237,14 @@ struct InferArgs {
 };
 
 // One hidden layer for the workgroup's 64 rows: Hout = BF16(GELU(Hin * W + b)).  Wave w owns
-// columns [w HID/4, (w+1) HID/4); all B fragments of the layer are fetched up front.
+// columns [w HID/4, (w+1) HID/4); fb holds its B fragments of the layer and bc its biases.
 template <int K, int HID>
-__device__ __forceinline__ void infer_layer(const uint16_t (*Hin)[kInferW + 8],
-                                            uint16_t (*Hout)[kInferW + 8], const uint16_t *wt,
-                                            const float *bias, int w, int lr, int lq) {
+__device__ __forceinline__ void infer_layer_w(const uint16_t (*Hin)[kInferW + 8],
+                                              uint16_t (*Hout)[kInferW + 8],
+                                              const bf16x4 (&fb)[K / 16][HID / 64],
+                                              const float (&bc)[HID / 64], int w, int lr,
+                                              int lq) {
   constexpr int NBW = HID / 64, KS = K / 16;
-  bf16x4 fb[KS][NBW];
-#pragma unroll
-  for (int ks = 0; ks < KS; ks++)
-#pragma unroll
-    for (int b = 0; b < NBW; b++)
-      fb[ks][b] = *(const bf16x4 *)(wt + (size_t)(w * (HID / 4) + b * 16 + lr) * K + ks * 16 + lq);
   f32x4 acc[4][NBW];
 #pragma unroll
   for (int m = 0; m < 4; m++)
@@ -268,12 +264,36 @@ __device__ __forceinline__ void infer_layer(const uint16_t (*Hin)[kInferW + 8],
 #pragma unroll
   for (int b = 0; b < NBW; b++) {
     const int col = w * (HID / 4) + b * 16 + lr;
-    const float bc = bias[col];
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) Hout[m * 16 + lq + v][col] = f2bf(gelu(acc[m][b][v] + bc));
+      for (int v = 0; v < 4; v++) Hout[m * 16 + lq + v][col] = f2bf(gelu(acc[m][b][v] + bc[b]));
   }
+}
+
+// The wave's B fragments and biases of one hidden layer.
+template <int K, int HID>
+__device__ __forceinline__ void infer_frags(bf16x4 (&fb)[K / 16][HID / 64],
+                                            float (&bc)[HID / 64], const uint16_t *wt,
+                                            const float *bias, int w, int lr, int lq) {
+#pragma unroll
+  for (int ks = 0; ks < K / 16; ks++)
+#pragma unroll
+    for (int b = 0; b < HID / 64; b++)
+      fb[ks][b] = *(const bf16x4 *)(wt + (size_t)(w * (HID / 4) + b * 16 + lr) * K + ks * 16 + lq);
+#pragma unroll
+  for (int b = 0; b < HID / 64; b++) bc[b] = bias[w * (HID / 4) + b * 16 + lr];
+}
+
+// All B fragments of the layer are fetched up front.
+template <int K, int HID>
+__device__ __forceinline__ void infer_layer(const uint16_t (*Hin)[kInferW + 8],
+                                            uint16_t (*Hout)[kInferW + 8], const uint16_t *wt,
+                                            const float *bias, int w, int lr, int lq) {
+  bf16x4 fb[K / 16][HID / 64];
+  float bc[HID / 64];
+  infer_frags<K, HID>(fb, bc, wt, bias, w, lr, lq);
+  infer_layer_w<K, HID>(Hin, Hout, fb, bc, w, lr, lq);
 }
 
 template <int DINP, int HID>
@@ -318,6 +338,77 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a) {
   }
 }
 
+// Persistent inference for NH = 1..3 hidden layers: each wave keeps its B fragments and biases
+// of every layer in registers for the whole launch, and the workgroup walks 64-row tiles; the
+// next tile's input rows are requested while the current tile's layers run.  The MFMA sequence
+// and the epilogue per output element are k_mlp_infer's, so the outputs are the same bit for
+// bit.  Hs[2] holds the input tile (its columns din..DINP-1 stay zero).
+template <int DINP, int HID, int NH>
+__global__ __launch_bounds__(256) void k_mlp_infer_p(InferArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t Hs[3][64][kInferW + 8];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lq = 4 * (lane >> 4);
+  constexpr int KH = HID / 16, NE = DINP / 4;
+  bf16x4 f0[DINP / 16][HID / 64], fh[NH][KH][HID / 64], fo[KH];
+  float c0[HID / 64], ch[NH][HID / 64];
+  infer_frags<DINP, HID>(f0, c0, a.wt[0], a.b[0], w, lr, lq);
+#pragma unroll
+  for (int l = 1; l < NH; l++) infer_frags<HID, HID>(fh[l - 1], ch[l - 1], a.wt[l], a.b[l], w, lr, lq);
+#pragma unroll
+  for (int ks = 0; ks < KH; ks++) fo[ks] = *(const bf16x4 *)(a.wt[NH] + (size_t)lr * HID + ks * 16 + lq);
+  const float co = lr < a.dout ? a.b[NH][lr] : 0.f;
+  for (int i = tid; i < 64 * (kInferW + 8); i += 256) (&Hs[2][0][0])[i] = 0;
+  // this thread's input slots: row tid / 4, columns tid % 4 + 4 e
+  const int xrow = tid >> 2, xc0 = tid & 3;
+  const int ntiles = (a.rows + 63) / 64;
+  float xr[NE];
+  auto fetch = [&](int tile) {
+    const int row = tile * 64 + xrow;
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      const int c = xc0 + 4 * e;
+      xr[e] = (tile < ntiles && row < a.rows && c < a.din) ? a.x[(size_t)row * a.din + c] : 0.f;
+    }
+  };
+  int tile = blockIdx.x;
+  fetch(tile);
+  __syncthreads();
+  for (; tile < ntiles; tile += gridDim.x) {
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      const int c = xc0 + 4 * e;
+      if (c < a.din) Hs[2][xrow][c] = f2bf(xr[e]);
+    }
+    fetch(tile + gridDim.x);
+    __syncthreads();
+    infer_layer_w<DINP, HID>(Hs[2], Hs[0], f0, c0, w, lr, lq);
+    __syncthreads();
+    int cur = 0;
+#pragma unroll
+    for (int l = 1; l < NH; l++) {
+      infer_layer_w<HID, HID>(Hs[cur], Hs[cur ^ 1], fh[l - 1], ch[l - 1], w, lr, lq);
+      __syncthreads();
+      cur ^= 1;
+    }
+    // output layer, columns 0..15; wave w takes rows 16w .. 16w+15
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KH; ks++) {
+      const bf16x4 fa = *(const bf16x4 *)&Hs[cur][w * 16 + lr][ks * 16 + lq];
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fo[ks], acc, 0, 0, 0);
+    }
+    if (lr < a.dout) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = tile * 64 + w * 16 + lq + v;
+        if (row < a.rows) a.y[(size_t)row * a.dout + lr] = acc[v] + co;
+      }
+    }
+    __syncthreads();  // every read of this tile's buffers precedes the next tile's writes
+  }
+}
+
 }  // namespace nb
 
 // ---------------------------------------------------------------------------------------------
@@ -349,6 +440,7 @@ struct nb_mlp {
   // transposed BF16 weights for the fused inference kernel, rebuilt when wver moves
   std::vector<uint16_t *> WbT;
   int64_t wver = 0, wt_ver = -1;
+  int infer_wgs = 0;  // resident workgroups of the persistent inference kernel (0: not yet known)
 };
 
 namespace {
@@ -614,8 +706,34 @@ int mlp_infer_device(nb_mlp *m, const float *x_dev, int64_t rows, float *y_dev,
     a.b[l] = l <= m->layers ? m->bvec[l] : nullptr;
   }
   a.y = y_dev;
-  const dim3 grid((unsigned)((rows + 63) / 64));
   const int key = (m->dinp == 64 ? 0 : 1) + (m->hid == 64 ? 0 : 2);
+  if (m->layers <= 3) {
+    // persistent: as many workgroups as stay resident, each walking tiles
+    const void *fn = nullptr;
+#define NB_INFER_P(D, H)                                                         \
+  fn = m->layers == 1   ? (const void *)k_mlp_infer_p<D, H, 1>                    \
+       : m->layers == 2 ? (const void *)k_mlp_infer_p<D, H, 2>                    \
+                        : (const void *)k_mlp_infer_p<D, H, 3>
+    switch (key) {
+      case 0: NB_INFER_P(64, 64); break;
+      case 1: NB_INFER_P(128, 64); break;
+      case 2: NB_INFER_P(64, 128); break;
+      default: NB_INFER_P(128, 128); break;
+    }
+#undef NB_INFER_P
+    if (m->infer_wgs == 0) {
+      int cus = 0, per = 0;
+      NB_MHIP(m, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->device));
+      NB_MHIP(m, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, 0));
+      m->infer_wgs = std::max(1, cus * std::max(1, per));
+    }
+    const int64_t tiles = (rows + 63) / 64;
+    const dim3 grid((unsigned)std::min<int64_t>(tiles, m->infer_wgs));
+    void *args[] = {&a};
+    NB_MHIP(m, hipLaunchKernel(fn, grid, dim3(256), args, 0, st));
+    return NB_OK;
+  }
+  const dim3 grid((unsigned)((rows + 63) / 64));
   switch (key) {
     case 0: hipLaunchKernelGGL((k_mlp_infer<64, 64>), grid, dim3(256), 0, st, a); break;
     case 1: hipLaunchKernelGGL((k_mlp_infer<128, 64>), grid, dim3(256), 0, st, a); break;

@@ -105,10 +105,13 @@ def test_device_forward_matches_oracle():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("d_in,hidden,layers,rows", [(17, 128, 1, 1000), (30, 128, 3, 4099),
-                                                     (30, 64, 2, 77)])
+                                                     (30, 64, 2, 77), (30, 128, 3, 70001),
+                                                     (30, 64, 4, 130)])
 def test_fused_inference_equals_layer_by_layer_forward(d_in, hidden, layers, rows):
     """The surrogate's one-kernel inference reproduces the GEMM-by-GEMM forward bit for bit
-    (same MFMA sequence, same epilogue), including ragged row counts."""
+    (same MFMA sequence, same epilogue), including ragged row counts.  Up to three hidden
+    layers run the persistent kernel (70,001 rows: several tiles per workgroup); four run the
+    one-tile-per-workgroup kernel."""
     from nbodygo_amd.mlp import MLP
     rng = np.random.default_rng(rows)
     x = rng.normal(size=(rows, d_in)).astype(np.float32)
