@@ -336,6 +336,7 @@ def collide_leg(a, pinned, steps):
         out[mode] = {"ms_per_step": el / steps * 1e3, "steps_per_s": steps / el,
                      "force_error_start": err_mlp, "pairs_total": st["pairs_total"],
                      "resolved_total": st["resolved_total"], "drift": drift(d0, d1),
+                     "flagged_chunks_per_step": st["fixup_tiles"] / steps,
                      "final": g.store()}
         if mode == "surrogate":
             out[mode]["error_without_mlp"] = err_base
