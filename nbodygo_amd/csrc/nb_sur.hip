@@ -742,8 +742,8 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   // A wave holds `copies` copies of the item's targets (the most, up to 8, that fit in 64
-  // lanes); copy cp takes every copies-th source of each tile and of the shell, and the copies'
-  // partial sums are combined in a fixed order at the end.
+  // lanes); copy cp takes a contiguous share of each tile and every copies-th shell cell, and
+  // the copies' partial sums are combined in a fixed order at the end.
   const int tcount = t_end - t_begin;
   const int copies = tcount > 32 ? 1 : tcount > 16 ? 2 : tcount > 8 ? 4 : 8;
   const int cwid = 64 / copies;  // lanes per copy
@@ -807,24 +807,27 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
         }
         wave_lds_sync();
         float px = 0, py = 0, pz = 0, r2min = FLT_MAX;
-        // sources j = cp, cp + copies, ...: four per step, padded with zero-mass entries (the
-        // padded count is a multiple of 4 copies and at most 64, the tile's size)
+        // the tile padded with zero-mass entries to a multiple of 4 copies (at most 64, the
+        // tile's size); copy cp takes the contiguous share [cp len, (cp + 1) len), four sources
+        // per step, with the same trip count on every lane
         const int q4 = 4 * copies;
         const int cntp = (cnt + q4 - 1) / q4 * q4;
+        const int len = cntp / copies;
+        const float4 *tc = tl + cp * len;
         if (COLL) {
           // the body's own entry (r^2 = eps^2) must not count as a candidate
           if (s0 < t_end && s0 + cnt > t_begin) {
             const int self = k - s0;
-            for (int j = cp; j < cnt; j += copies) {
+            const int j1 = min(cp * len + len, cnt);
+            for (int j = cp * len; j < j1; j++) {
               const float keep = r2min;
               near_pair_min(tl[j], me, eps2, px, py, pz, r2min);
               if (j == self) r2min = keep;
             }
           } else {
             // zero-mass padding may only add candidates (an exact pass over the real sources)
-            for (int j = cp; j < cntp; j += q4) {
-              const float4 b0 = tl[j], b1 = tl[j + copies], b2 = tl[j + 2 * copies],
-                           b3 = tl[j + 3 * copies];
+            for (int j = 0; j < len; j += 4) {
+              const float4 b0 = tc[j], b1 = tc[j + 1], b2 = tc[j + 2], b3 = tc[j + 3];
               near_pair_min(b0, me, eps2, px, py, pz, r2min);
               near_pair_min(b1, me, eps2, px, py, pz, r2min);
               near_pair_min(b2, me, eps2, px, py, pz, r2min);
@@ -834,9 +837,8 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
           // the chunk's minimum over all copies, for the target's one decision below
           for (int o = cwid; o < 64; o <<= 1) r2min = fminf(r2min, __shfl_xor(r2min, o));
         } else {
-          for (int j = cp; j < cntp; j += q4) {
-            const float4 b0 = tl[j], b1 = tl[j + copies], b2 = tl[j + 2 * copies],
-                         b3 = tl[j + 3 * copies];
+          for (int j = 0; j < len; j += 4) {
+            const float4 b0 = tc[j], b1 = tc[j + 1], b2 = tc[j + 2], b3 = tc[j + 3];
             near_pair(b0, me, eps2, px, py, pz);
             near_pair(b1, me, eps2, px, py, pz);
             near_pair(b2, me, eps2, px, py, pz);

@@ -3,11 +3,12 @@
 config 4   Plummer sphere of N = 262,144 (a = 1, M = 1, G = 1, eps = 0.01): the learned far-field
            surrogate (32^3 stretched grid, shell 2, MLP trained on direct-sum rows of 200 seeded
            sims of the same size) against the oracle's FP64 softened direct sum on 1,024 sampled
-           targets: global relative error <= 1e-3 and per-body p99 <= P99_C4 (2e-3); then 10
-           checked cycles with no fallback.
+           targets: global relative error <= 1e-3, per-body p99 <= P99_C4 (2e-3) and per-body
+           max <= MAX_C4 (3e-2); then 10 checked cycles with no fallback.
 config 5   two Plummer spheres of 524,288 bodies each (masses 1 : 3, 10 a apart, approaching at
            0.5), elastic collisions, radii 0.05 m^(1/3), surrogate on a 64^3 grid with shell 3,
-           10 checked cycles: sampled force error against the oracle <= 1e-3, each cycle's pair
+           10 checked cycles: sampled force error against the oracle <= 1e-3 (p99 <= P99_C5,
+           max <= MAX_C5), each cycle's pair
            list identical to the oracle's detection (cell lists, pinned to the brute-force
            detector) on the state the cycle started from, and FP64 energy and momentum drift
            within the bounds below.
@@ -24,13 +25,18 @@ pytestmark = pytest.mark.gpu
 
 NT = 16
 P99_C4 = 2e-3          # per-body 99th percentile of |da| / |a|, config 4
+MAX_C4 = 3e-2          # largest per-body |da| / |a| of the sample, config 4 (measured 1e-2 on
+                       # 4,096 targets: bodies near the centre, where |a| -> 0)
+P99_C5 = 4e-3          # the same for config 5 (measured 2.2e-3 to 2.5e-3)
+MAX_C5 = 5e-2          # (measured 8e-3 to 1.3e-2)
 E_DRIFT_C5 = 1e-6      # |E_10 - E_0| / |E_0| over 10 cycles, config 5
 P_DRIFT_C5 = 5e-6      # |P_10 - P_0| / sum m |v| over 10 cycles, config 5
 
 
 def errors(a, ref):
     per = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    return float(np.linalg.norm(a - ref) / np.linalg.norm(ref)), float(np.percentile(per, 99))
+    return (float(np.linalg.norm(a - ref) / np.linalg.norm(ref)), float(np.percentile(per, 99)),
+            float(per.max()))
 
 
 def train(g, make, sims, rows, seed0, **sur):
@@ -57,12 +63,13 @@ def test_config4_surrogate_at_size(oracle):
         s0 = g.sur_stats()
         g.step(1e-3, 1.0, nsteps=10)
         s1 = g.sur_stats()
-    glob, p99 = errors(a, ref)
-    print("config 4: global %.3e p99 %.3e, %d checks, %d fallbacks"
-          % (glob, p99, s1["checks"] - s0["checks"], s1["fallbacks"] - s0["fallbacks"]),
+    glob, p99, mx = errors(a, ref)
+    print("config 4: global %.3e p99 %.3e max %.3e, %d checks, %d fallbacks"
+          % (glob, p99, mx, s1["checks"] - s0["checks"], s1["fallbacks"] - s0["fallbacks"]),
           flush=True)
     assert glob <= 1e-3
     assert p99 <= P99_C4
+    assert mx <= MAX_C4
     assert s1["checks"] - s0["checks"] == 10
     assert s1["fallbacks"] == s0["fallbacks"]
 
@@ -95,14 +102,16 @@ def test_config5_colliding_spheres_at_size(oracle):
             npairs += len(pairs)
         d1 = g.diagnostics()
         stats = g.sur_stats()
-    glob, p99 = errors(a, ref)
+    glob, p99, mx = errors(a, ref)
     e0, e1 = d0["kinetic"] + d0["potential"], d1["kinetic"] + d1["potential"]
     e_drift = abs(e1 - e0) / abs(e0)
     p_drift = float(np.linalg.norm(np.asarray(d1["momentum"]) - d0["momentum"]) / pscale)
-    print("config 5: force error %.3e (p99 %.3e), %d pairs in 10 cycles, energy drift %.2e, "
-          "momentum drift %.2e, %d fallbacks" % (glob, p99, npairs, e_drift, p_drift,
-                                                 stats["fallbacks"]), flush=True)
+    print("config 5: force error %.3e (p99 %.3e, max %.3e), %d pairs in 10 cycles, energy drift "
+          "%.2e, momentum drift %.2e, %d fallbacks" % (glob, p99, mx, npairs, e_drift, p_drift,
+                                                       stats["fallbacks"]), flush=True)
     assert glob <= 1e-3
+    assert p99 <= P99_C5
+    assert mx <= MAX_C5
     assert npairs > 0
     assert e_drift <= E_DRIFT_C5
     assert p_drift <= P_DRIFT_C5
