@@ -326,9 +326,14 @@ int nb_batch_store(nb_batch *b, double *x, double *y, double *z, double *vx, dou
 /* Kernel shape of the batched force.  Work items are blocks of 64 x IPL targets of one
  * simulation; `waves` (1, 2 or 4) wavefronts share an item, splitting its sources; each wave
  * consumes its sources in Kahan tiles of `tile` (64, 128 or 256) sources.  IPL and tile are chosen
- * per size bucket (N < 1024, < 4096, >= 4096), waves for the whole launch.
- * nb_batch_autotune times every (IPL, tile) per bucket for every waves choice on the loaded
- * simulations and keeps the fastest; ipl3 (may be NULL) receives the per-bucket IPL.
+ * per size bucket (N < 1024, < 4096, >= 4096), waves for the whole launch.  When every bucket
+ * has the same (IPL, tile) the launch runs a kernel specialised to that shape (fewer registers
+ * and LDS, so more waves per SIMD); otherwise one kernel reads each item's shape.  Both give the
+ * same forces bit for bit for a given (waves, tile).
+ * nb_batch_autotune, for every waves choice, finds the fastest (IPL, tile) of each bucket alone,
+ * then times full launches of those per-bucket winners together and of every single (IPL,
+ * tile) for all buckets, and keeps the fastest full launch; ipl3 (may be NULL) receives the
+ * per-bucket IPL.
  * Tuning words: cfg7 = {waves, ipl0, tile0, ipl1, tile1, ipl2, tile2}. */
 int nb_batch_get_shapes(nb_batch *b, int32_t *ipl3);
 int nb_batch_set_shapes(nb_batch *b, const int32_t *ipl3);

@@ -68,10 +68,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // its own LDS buffer (one 16-byte load per lane per 64 sources), reads it back as wave-uniform
 // broadcasts, keeps plain FP32 partial sums within the tile and Kahan-adds them at the tile's
 // end.  With W > 1 the waves' FP64 results are added in wave order through LDS.
-template <int W, int IPL, int TILE>
+template <int W, int IPL, int TILE, int NT, int NS>
 __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, double *acc,
-                           int npad, float eps2_s, double (*lds)[3][kBatchMaxTargets],
-                           float4 (*tiles_lds)[kBatchMaxTile]) {
+                           int npad, float eps2_s, double (*lds)[3][NT],
+                           float4 (*tiles_lds)[NS]) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;          // the workgroup's wave: its LDS tile
   const int w = W == 1 ? 0 : wave;            // its share of the item's sources
@@ -98,10 +98,7 @@ __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, d
     for (int k = 0; k < IPL; k++) px[k] = py[k] = pz[k] = 0.f;
     for (int q = lane; q < cnt; q += 64) tl[q] = js[it.s0 + t0 + q];
     wave_lds_sync();
-    // cnt is a multiple of 64: sixteen LDS reads per iteration share one base address
-#pragma unroll 16
-    for (int jj = 0; jj < cnt; jj++) {
-      const float4 b = tl[jj];
+    auto interact = [&](const float4 b) {
       float dx[IPL], dy[IPL], dz[IPL], inv[IPL];
 #pragma unroll
       for (int k = 0; k < IPL; k++) {
@@ -122,6 +119,22 @@ __device__ void batch_item(const BatchItem &it, const float4 *__restrict__ js, d
         py[k] = __builtin_fmaf(dy[k], s, py[k]);
         pz[k] = __builtin_fmaf(dz[k], s, pz[k]);
       }
+    };
+    // cnt is a multiple of 64.  One target per lane: eight sources per step are read first
+    // (eight ds_read_b128 in flight, one base address, 63 VGPRs).  Two or four: sixteen sources
+    // per unrolled iteration share one base address (the compiler keeps fewer reads ahead, which
+    // holds the registers of those shapes lowest).
+    if constexpr (IPL == 1) {
+      for (int j8 = 0; j8 < cnt; j8 += 8) {
+        float4 b8[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) b8[u] = tl[j8 + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) interact(b8[u]);
+      }
+    } else {
+#pragma unroll 16
+      for (int jj = 0; jj < cnt; jj++) interact(tl[jj]);
     }
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
@@ -172,21 +185,27 @@ __device__ __forceinline__ void batch_item_tile(const BatchItem &it, const float
                                                 double *acc, int npad, float eps2,
                                                 double (*lds)[3][kBatchMaxTargets],
                                                 float4 (*tiles_lds)[kBatchMaxTile]) {
-  if (it.tile == 256) batch_item<W, IPL, 256>(it, js, acc, npad, eps2, lds, tiles_lds);
-  else if (it.tile == 128) batch_item<W, IPL, 128>(it, js, acc, npad, eps2, lds, tiles_lds);
-  else batch_item<W, IPL, 64>(it, js, acc, npad, eps2, lds, tiles_lds);
+  constexpr int NT = kBatchMaxTargets, NS = kBatchMaxTile;
+  if (it.tile == 256) batch_item<W, IPL, 256, NT, NS>(it, js, acc, npad, eps2, lds, tiles_lds);
+  else if (it.tile == 128) batch_item<W, IPL, 128, NT, NS>(it, js, acc, npad, eps2, lds, tiles_lds);
+  else batch_item<W, IPL, 64, NT, NS>(it, js, acc, npad, eps2, lds, tiles_lds);
 }
 
 // Persistent grid: each workgroup (W waves) takes items from the queue until it is exhausted, so
 // the grid drains on its own.  W = 1: the four waves of a 256-thread workgroup are independent
-// and each takes its own items.
-template <int W>
+// and each takes its own items.  IPL = 0 reads each item's shape; a launch whose items all share
+// one shape uses the kernel specialised to it (IPL, TILE > 0), whose registers are sized for
+// that shape alone.
+template <int W, int IPL, int TILE>
 __global__ __launch_bounds__(256) void k_batch_force(const BatchItem *__restrict__ items,
                                                      int nitems, BatchQueue *queue,
                                                      const float4 *__restrict__ js, double *acc,
                                                      int npad, float eps2) {
-  __shared__ double lds[W > 1 ? W : 1][3][kBatchMaxTargets];
-  __shared__ float4 tiles_lds[W > 1 ? W : 4][kBatchMaxTile];
+  // LDS sized for the shape: the specialised kernel's item targets and tile only
+  constexpr int NT = IPL > 0 ? 64 * IPL : kBatchMaxTargets;
+  constexpr int NS = TILE > 0 ? TILE : kBatchMaxTile;
+  __shared__ double lds[W > 1 ? W : 1][3][NT];
+  __shared__ float4 tiles_lds[W > 1 ? W : 4][NS];
   __shared__ int cur[4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (;;) {
@@ -204,12 +223,17 @@ __global__ __launch_bounds__(256) void k_batch_force(const BatchItem *__restrict
     }
     if (q >= nitems) break;
     const BatchItem it = items[q];
-    if (it.ipl == 4)
-      batch_item_tile<W, 4>(it, js, acc, npad, eps2, lds, tiles_lds);
-    else if (it.ipl == 2)
-      batch_item_tile<W, 2>(it, js, acc, npad, eps2, lds, tiles_lds);
-    else
-      batch_item_tile<W, 1>(it, js, acc, npad, eps2, lds, tiles_lds);
+    if constexpr (IPL > 0) {
+      batch_item<W, (IPL > 0 ? IPL : 1), (TILE > 0 ? TILE : 64), NT, NS>(it, js, acc, npad, eps2,
+                                                                         lds, tiles_lds);
+    } else {
+      if (it.ipl == 4)
+        batch_item_tile<W, 4>(it, js, acc, npad, eps2, lds, tiles_lds);
+      else if (it.ipl == 2)
+        batch_item_tile<W, 2>(it, js, acc, npad, eps2, lds, tiles_lds);
+      else
+        batch_item_tile<W, 1>(it, js, acc, npad, eps2, lds, tiles_lds);
+    }
   }
 }
 
@@ -244,6 +268,7 @@ struct nb_batch {
   int bucket_ipl[kBuckets] = {2, 4, 4};
   int bucket_tile[kBuckets] = {64, 64, 64};
   int waves = 4;  // waves per work item (1, 2 or 4)
+  int items_ipl = 0, items_tile = 0;  // the shape every current item shares (0: mixed)
   int cus = 0;
   bool use_graph = true;
   hipGraph_t graph = nullptr;
@@ -324,6 +349,13 @@ int build_items(nb_batch *b, int only_bucket, int force_ipl, int force_tile = 0)
     return (int64_t)a.tn * a.sn > (int64_t)c.tn * c.sn;
   });
   b->nitems = (int)v.size();
+  b->items_ipl = v.empty() ? 0 : v[0].ipl;
+  b->items_tile = v.empty() ? 0 : v[0].tile;
+  for (const BatchItem &it : v)
+    if (it.ipl != b->items_ipl || it.tile != b->items_tile) {
+      b->items_ipl = b->items_tile = 0;
+      break;
+    }
   if (!v.empty())
     NB_BHIP(b, hipMemcpyAsync(b->items, v.data(), v.size() * sizeof(BatchItem),
                               hipMemcpyHostToDevice, b->stream));
@@ -332,20 +364,40 @@ int build_items(nb_batch *b, int only_bucket, int force_ipl, int force_tile = 0)
   return NB_OK;
 }
 
+template <int W>
+static const void *batch_kernel(int ipl, int tile) {
+  switch (ipl * 1000 + tile) {
+    case 1064: return (const void *)k_batch_force<W, 1, 64>;
+    case 1128: return (const void *)k_batch_force<W, 1, 128>;
+    case 1256: return (const void *)k_batch_force<W, 1, 256>;
+    case 2064: return (const void *)k_batch_force<W, 2, 64>;
+    case 2128: return (const void *)k_batch_force<W, 2, 128>;
+    case 2256: return (const void *)k_batch_force<W, 2, 256>;
+    case 4064: return (const void *)k_batch_force<W, 4, 64>;
+    case 4128: return (const void *)k_batch_force<W, 4, 128>;
+    case 4256: return (const void *)k_batch_force<W, 4, 256>;
+    default: return (const void *)k_batch_force<W, 0, 0>;
+  }
+}
+
 hipError_t launch_batch_force(nb_batch *b) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->queue);
-  const float eps2 = (float)(b->p.eps * b->p.eps);
+  float eps2 = (float)(b->p.eps * b->p.eps);
+  // items of one shape: the kernel specialised to it; otherwise the per-item dispatch
+  const int ipl = b->items_ipl, tile = b->items_tile;
   // 32 waves per CU: 8 workgroups of 4 waves (W = 4, and W = 1 with independent waves), 16 of 2
-  if (b->waves == 2)
-    hipLaunchKernelGGL(k_batch_force<2>, dim3((unsigned)(b->cus * 16)), dim3(128), 0, b->stream,
-                       b->items, b->nitems, b->queue, b->js, b->acc, (int)b->n, eps2);
-  else if (b->waves == 1)
-    hipLaunchKernelGGL(k_batch_force<1>, dim3((unsigned)(b->cus * 8)), dim3(256), 0, b->stream,
-                       b->items, b->nitems, b->queue, b->js, b->acc, (int)b->n, eps2);
-  else
-    hipLaunchKernelGGL(k_batch_force<4>, dim3((unsigned)(b->cus * 8)), dim3(256), 0, b->stream,
-                       b->items, b->nitems, b->queue, b->js, b->acc, (int)b->n, eps2);
-  return hipGetLastError();
+  const void *fn = b->waves == 2   ? batch_kernel<2>(ipl, tile)
+                   : b->waves == 1 ? batch_kernel<1>(ipl, tile)
+                                   : batch_kernel<4>(ipl, tile);
+  const dim3 grid((unsigned)(b->cus * (b->waves == 2 ? 16 : 8)));
+  const dim3 block(b->waves == 2 ? 128 : 256);
+  const BatchItem *items = b->items;
+  int nitems = b->nitems, npad = (int)b->n;
+  BatchQueue *queue = b->queue;
+  const float4 *js = b->js;
+  double *acc = b->acc;
+  void *args[] = {&items, &nitems, &queue, &js, &acc, &npad, &eps2};
+  return hipLaunchKernel(fn, grid, block, args, 0, b->stream);
 }
 
 hipError_t enqueue_cycle(nb_batch *b, double dt, double R) {
@@ -630,8 +682,10 @@ int nb_batch_set_tuning(nb_batch *b, const int32_t *cfg7) {
   return b->items ? build_items(b, -1, 0) : NB_OK;
 }
 
-// For each waves-per-item choice, every bucket's items alone under each (IPL, tile); the waves
-// choice whose best per-bucket times sum lowest is kept, with those per-bucket shapes.
+// For each waves-per-item choice: every bucket's items alone under each (IPL, tile), then two
+// kinds of whole-batch candidate, each timed as a full launch: the per-bucket winners together
+// (the per-item dispatch kernel, or the specialised one when they coincide) and every single
+// (IPL, tile) for all items (the specialised kernel).  The fastest full launch is kept.
 int nb_batch_autotune(nb_batch *b, int32_t reps, int32_t *ipl3) {
   if (!b) return NB_E_INVALID;
   if (!b->items) return bfail(b, NB_E_STATE, "nb_batch_autotune: nothing loaded");
@@ -640,12 +694,22 @@ int nb_batch_autotune(nb_batch *b, int32_t reps, int32_t *ipl3) {
   hipEvent_t e0, e1;
   NB_BHIP(b, hipEventCreate(&e0));
   NB_BHIP(b, hipEventCreate(&e1));
+  // mean ms of `reps` launches of the current items, after one warm-up launch
+  auto time_launch = [&](float &ms) -> int {
+    NB_BHIP(b, launch_batch_force(b));
+    NB_BHIP(b, hipEventRecord(e0, b->stream));
+    for (int r = 0; r < reps; r++) NB_BHIP(b, launch_batch_force(b));
+    NB_BHIP(b, hipEventRecord(e1, b->stream));
+    NB_BHIP(b, hipEventSynchronize(e1));
+    NB_BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+    ms /= (float)reps;
+    return NB_OK;
+  };
   float best_total = 1e30f;
   int best_w = b->waves, best_ipl[kBuckets], best_tile[kBuckets];
   for (int k = 0; k < kBuckets; k++) best_ipl[k] = b->bucket_ipl[k], best_tile[k] = b->bucket_tile[k];
   for (int w : {1, 2, 4}) {
     b->waves = w;
-    float total = 0;
     int ipl_w[kBuckets], tile_w[kBuckets];
     for (int bk = 0; bk < kBuckets; bk++) {
       float best = 1e30f;
@@ -656,26 +720,37 @@ int nb_batch_autotune(nb_batch *b, int32_t reps, int32_t *ipl3) {
           int rc = build_items(b, bk, ipl, tile);
           if (rc) return rc;
           if (b->nitems == 0) break;
-          NB_BHIP(b, launch_batch_force(b));  // warm-up
-          NB_BHIP(b, hipEventRecord(e0, b->stream));
-          for (int r = 0; r < reps; r++) NB_BHIP(b, launch_batch_force(b));
-          NB_BHIP(b, hipEventRecord(e1, b->stream));
-          NB_BHIP(b, hipEventSynchronize(e1));
           float ms = 0;
-          NB_BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+          if ((rc = time_launch(ms))) return rc;
           if (ms < best) {
             best = ms;
             ipl_w[bk] = ipl;
             tile_w[bk] = tile;
           }
         }
-      if (best < 1e30f) total += best;
     }
-    if (total < best_total) {
-      best_total = total;
+    // the per-bucket winners together
+    for (int k = 0; k < kBuckets; k++) b->bucket_ipl[k] = ipl_w[k], b->bucket_tile[k] = tile_w[k];
+    int rc = build_items(b, -1, 0);
+    if (rc) return rc;
+    float ms = 0;
+    if ((rc = time_launch(ms))) return rc;
+    if (ms < best_total) {
+      best_total = ms;
       best_w = w;
       for (int k = 0; k < kBuckets; k++) best_ipl[k] = ipl_w[k], best_tile[k] = tile_w[k];
     }
+    // one shape for all items
+    for (int ipl : {1, 2, 4})
+      for (int tile : {64, 128, 256}) {
+        if ((rc = build_items(b, -1, ipl, tile))) return rc;
+        if ((rc = time_launch(ms))) return rc;
+        if (ms < best_total) {
+          best_total = ms;
+          best_w = w;
+          for (int k = 0; k < kBuckets; k++) best_ipl[k] = ipl, best_tile[k] = tile;
+        }
+      }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

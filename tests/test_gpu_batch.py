@@ -22,12 +22,16 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("waves,tile", [(4, 64), (1, 256), (2, 128)])
-def test_batch_forces_match_oracle_per_sim(oracle, waves, tile):
+@pytest.mark.parametrize("waves,buckets", [
+    (4, [(2, 64), (4, 64), (4, 64)]), (1, [(2, 256), (4, 256), (4, 256)]),
+    (2, [(2, 128), (4, 128), (4, 128)]),
+    # one shape for every bucket: the kernel specialised to it
+    (4, [(1, 256)] * 3), (2, [(2, 64)] * 3), (1, [(4, 128)] * 3)])
+def test_batch_forces_match_oracle_per_sim(oracle, waves, buckets):
     sims = sims_mixed()
     with NBodyBatch(G=1.0, eps=0.01) as bt:
         bt.load(sims)
-        bt.set_tuning(waves, [(2, tile), (4, tile), (4, tile)])
+        bt.set_tuning(waves, buckets)
         f = bt.forces()
     off = bt.offsets
     for k, s in enumerate(sims):
@@ -80,6 +84,11 @@ def test_shapes_do_not_change_results():
             out.append(bt.forces())
     np.testing.assert_array_equal(out[0], out[1])
     np.testing.assert_array_equal(out[0], out[2])
+    # one IPL per bucket runs the kernel specialised to it; mixed IPLs the per-item dispatch
+    with NBodyBatch(G=1.0, eps=0.01) as bt:
+        bt.load(sims)
+        bt.set_shapes([1, 2, 4])
+        np.testing.assert_array_equal(bt.forces(), out[0])
     for waves in (1, 2, 4):
         for tile in (64, 128, 256):
             with NBodyBatch(G=1.0, eps=0.01) as bt:
