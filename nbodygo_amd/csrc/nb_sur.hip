@@ -1085,8 +1085,8 @@ __global__ __launch_bounds__(256) void k_sur_check(const int32_t *sidx, int S,
                                                    double tol_body, double quantile,
                                                    uint32_t *body, uint32_t *flag, double *err) {
   __shared__ double sn[256], sd[256];
-  __shared__ int hist[256];
-  __shared__ uint32_t s_prefix;
+  __shared__ int hist[256], suf[257];
+  __shared__ uint32_t s_prefix, s_digit;
   __shared__ int s_k;
   const int nact = cstart[ncell];
   double num = 0, den = 0;
@@ -1131,15 +1131,28 @@ __global__ __launch_bounds__(256) void k_sur_check(const int32_t *sidx, int S,
       const uint32_t u = body[s];
       if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1);
     }
+    // the digit of the k-th largest among the values with this prefix: the largest b whose
+    // suffix count suf[b] = sum over b' >= b of hist[b'] reaches k (b = 0 when none above does)
+    suf[threadIdx.x] = hist[threadIdx.x];
+    if (threadIdx.x == 0) {
+      suf[256] = 0;
+      s_digit = 0;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {  // the digit of the k-th largest among the values with this prefix
-      int k = s_k, b = 255;
-      for (; b > 0; b--) {
-        if (hist[b] >= k) break;
-        k -= hist[b];
-      }
-      s_k = k;
-      s_prefix = prefix | ((uint32_t)b << shift);
+    for (int off = 1; off < 256; off <<= 1) {
+      const int add = threadIdx.x + off < 256 ? suf[threadIdx.x + off] : 0;
+      __syncthreads();
+      suf[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const int k = s_k;
+    const int b = (int)threadIdx.x;
+    if (b > 0 && suf[b] >= k && suf[b + 1] < k) s_digit = (uint32_t)b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t d = s_digit;
+      s_k = k - suf[d + 1];
+      s_prefix = prefix | (d << shift);
     }
     mask |= 255u << shift;
     __syncthreads();
