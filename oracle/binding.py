@@ -6,6 +6,7 @@ GPU path without sharing code with it.
 import ctypes as C
 import os
 import subprocess
+import time
 
 import numpy as np
 
@@ -87,6 +88,9 @@ def lib():
         L.or_bh_force.argtypes = [sp, C.POINTER(OrBhParams), _dp, _dp, _dp,
                                   C.POINTER(C.c_int32), C.POINTER(OrBhInfo)]
         L.or_bh_force.restype = C.c_int
+        L.or_soa_soft.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                  C.c_int64, C.c_int64, _dp, _dp, _dp, C.c_int]
+        L.or_soa_soft.restype = None
         _lib = L
     return _lib
 
@@ -163,6 +167,19 @@ def force_soft(st, G, eps, cutoff=False, idx=None, nthreads=8, long_double=False
         lib().or_force_soft(C.byref(st.s), G, eps, int(cutoff), ip, k,
                             *(f[c].ctypes.data_as(_dp) for c in range(3)), nthreads)
     return f.T.copy()
+
+
+def soa_soft(x, y, z, m, G, eps, t0=0, t1=None, nthreads=8):
+    """or_soa_soft: the benchmark's optimised CPU line (vectorised, threaded FP64 softened sum
+    over SoA arrays) for targets [t0, t1); returns (accelerations [k x 3], seconds)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, m)]
+    n = len(arrs[0])
+    t1 = n if t1 is None else t1
+    out = np.zeros((3, t1 - t0))
+    t = time.perf_counter()
+    lib().or_soa_soft(n, *(a.ctypes.data_as(_dp) for a in arrs), G, eps, t0, t1,
+                      *(out[c].ctypes.data_as(_dp) for c in range(3)), nthreads)
+    return out.T.copy(), time.perf_counter() - t
 
 
 def detect(st):

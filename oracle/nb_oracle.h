@@ -82,6 +82,13 @@ void or_force_ref(const or_state *s, double G, const int64_t *idx, int64_t nidx,
 void or_force_soft(const or_state *s, double G, double eps, int cutoff, const int64_t *idx,
                    int64_t nidx, double *fx, double *fy, double *fz, int nthreads);
 
+/* The benchmark's stronger CPU line (cpu_soa.c): the same softened sum over SoA arrays,
+ * vectorised and OpenMP-threaded, for targets [t0, t1); G applied; not the reference's
+ * algorithm and not in the reference's operation order. */
+void or_soa_soft(int64_t n, const double *x, const double *y, const double *z, const double *m,
+                 double G, double eps, int64_t t0, int64_t t1, double *ax, double *ay,
+                 double *az, int nthreads);
+
 /* Same sums in long double (80-bit x87) for pinning the double versions. */
 void or_force_soft_ld(const or_state *s, double G, double eps, int cutoff, const int64_t *idx,
                       int64_t nidx, double *fx, double *fy, double *fz);

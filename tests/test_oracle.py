@@ -371,3 +371,17 @@ def test_bench_cycles_equals_the_serial_cycle(oracle, threads):
         np.testing.assert_array_equal(c.a[k], a.a[k])
     np.testing.assert_array_equal(c.flags, a.flags)
     assert not (c.flags[5] & oracle.F_EXISTS)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_soa_line_matches_the_softened_sum(oracle, threads):
+    """The benchmark's optimised CPU line (vectorised, threaded, its own operation order) gives
+    the oracle's FP64 softened accelerations to FP64 rounding, on any target slice."""
+    b = gen.plummer(1500, seed=9)
+    st = oracle.State.from_bodies(b)
+    ref = oracle.force_soft(st, 2.0, 0.05, nthreads=4) / b.m[:, None]
+    a, _ = oracle.soa_soft(b.x, b.y, b.z, b.m, 2.0, 0.05, nthreads=threads)
+    per = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert per.max() < 1e-12
+    part, _ = oracle.soa_soft(b.x, b.y, b.z, b.m, 2.0, 0.05, t0=100, t1=777, nthreads=threads)
+    np.testing.assert_array_equal(part, a[100:777])
