@@ -863,19 +863,23 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
     }
     base += rlen;
   }
-  // multipole shell, Chebyshev distance 2..K: the nonempty cells of the (2K+1)^3 box, listed by
-  // ballot (one box position per lane, 64 per round) in box order and dealt round robin to the
-  // waves; each wave stages its records in LDS (one per lane) and reads them back as broadcasts.
+  // multipole shell, Chebyshev distance 2..K: wave w takes the box positions w, w + 4, ... of
+  // the (2K+1)^3 box, one per lane and 64 per round; the nonempty ones are staged in LDS (one per
+  // lane) by ballot and read back as broadcasts.  Positions are decoded with a float reciprocal:
+  // for p < (2K+1)^3 <= 2197 the product (p + 0.5) / D lies at least 0.5 / D from an integer,
+  // far beyond its rounding, so the truncation is the exact quotient.
   {
     const int K = a.K, D = 2 * K + 1, box = D * D * D;
+    const float invD = 1.f / (float)D;
     float sx = 0, sy = 0, sz = 0;
-    int q = 0;  // list entries before this round
-    for (int p0 = 0; p0 < box; p0 += 64) {
-      const int p = p0 + lane;
+    for (int p0 = 0; p0 < box; p0 += 256) {
+      const int p = p0 + 4 * lane + wave;
       bool ok = false;
       int cell = 0;
       if (p < box) {
-        const int dx = p % D - K, dy = (p / D) % D - K, dz = p / (D * D) - K;
+        const int pq = (int)(((float)p + 0.5f) * invD);     // p / D
+        const int pr = (int)(((float)pq + 0.5f) * invD);    // p / D^2
+        const int dx = p - pq * D - K, dy = pq - pr * D - K, dz = pr - K;
         const int x = cx + dx, y = cy + dy, z = cz + dz;
         if (max(abs(dx), max(abs(dy), abs(dz))) >= 2 && x >= 0 && x < G && y >= 0 && y < G &&
             z >= 0 && z < G) {
@@ -884,24 +888,19 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
         }
       }
       const uint64_t okm = __ballot(ok);
-      const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(okm >> 32),
+      const int slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(okm >> 32),
                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)okm, 0u));
-      const bool mine = ok && (((q + rank) & 3) == wave);
-      const uint64_t mm = __ballot(mine);
-      const int slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(mm >> 32),
-                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mm, 0u));
-      if (mine) {
+      if (ok) {
         const float4 *r = (const float4 *)(a.crec + (size_t)cell * kCellRec);
         rl[slot][0] = r[0];
         rl[slot][1] = r[1];
         rl[slot][2] = r[2];
       }
       wave_lds_sync();
-      const int nsel = __builtin_popcountll(mm);
+      const int nsel = __builtin_popcountll(okm);
       for (int j = cp; j < nsel; j += copies)
         quad_field(rl[j][0], rl[j][1], rl[j][2], me.x, me.y, me.z, eps2, sx, sy, sz);
       wave_lds_sync();
-      q += __builtin_popcountll(okm);
     }
     ax += sx;
     ay += sy;
