@@ -14,7 +14,8 @@
 // Kahan-compensated partial sums are combined through LDS in wave order, so results do not
 // depend on scheduling (with W = 1 each wave takes items of its own).  The target-block width
 // (IPL) and the Kahan tile (64, 128 or 256 sources) are chosen per size bucket, W per launch, by
-// nb_batch_autotune.
+// nb_batch_autotune.  When every item has the same (IPL, tile) the launch uses the kernel
+// specialised to that shape, whose registers and LDS are sized for it alone.
 //
 // One nb_batch_step call runs nsteps cycles as one hipGraph launch: per cycle a fill kernel resets
 // the queue (no memset nodes), then the force kernel, then the update kernel.
