@@ -60,15 +60,29 @@ struct BhCounters {
 };
 
 // A child as the force walk needs it, eight per node: centre of mass and mass (a leaf's from
-// updateMass from zero), half (-1 marks a bucket), the child's code (-1 empty, <= -2 the leaf of
-// sorted body -2 - code, else a node), and the walk's links: `first` (slot of its first live
-// child, for a node that is opened) and `skip` (slot visited next once this subtree is done;
-// -1 ends the walk).  A slot is live when its code is not -1 and its mass is > 0.  One slot is
-// one 64-byte line.
-struct alignas(64) BhSlot {
-  double x, y, z, m, half;
-  int32_t code, first, skip, pad;
+// updateMass from zero), the child's code (-1 empty, <= -2 the leaf of sorted body -2 - code,
+// else a node), the walk's links: `first` (slot of its first live child, for a node that is
+// opened) and `skip` (slot visited next once this subtree is done; -1 ends the walk), and `dep`:
+// a node's depth (its half is the root's halved dep times, exactly as the build halves it), -1
+// for a bucket, 0 for a leaf or an empty slot.  A slot is live when its code is not -1 and its
+// mass is > 0.  48 bytes: three 16-byte loads per visit (the walk's reads of the slots bound
+// it), the half coming from a per-depth table.
+struct alignas(16) BhSlot {
+  double x, y, z, m;
+  int32_t code, first, skip, dep;
 };
+static_assert(sizeof(BhSlot) == 48, "three 16-byte words");
+
+// 2 half of each depth (the root's half, halved once per level: exact, as in the build), for
+// d = 0..kBhLevels, into a workgroup's LDS table.
+__device__ __forceinline__ void bh_half2_table(const double4 *nbox, double *h2s) {
+  if (threadIdx.x <= kBhLevels) {
+    double h = nbox[0].w;
+    for (int d = 0; d < (int)threadIdx.x; d++) h = h / 2;
+    h2s[threadIdx.x] = 2 * h;
+  }
+  __syncthreads();
+}
 
 struct Bh {
   nb_bh_params p{};
@@ -382,9 +396,9 @@ __global__ __launch_bounds__(256) void k_bh_mass(BhMassArgs a) {
       BhSlot o;
       o.code = code;
       o.first = o.skip = -1;
-      o.pad = 0;
+      o.dep = 0;
       if (code == -1) {
-        o.x = o.y = o.z = o.m = o.half = 0.0;
+        o.x = o.y = o.z = o.m = 0.0;
         sl[c] = o;
         continue;
       }
@@ -394,14 +408,13 @@ __global__ __launch_bounds__(256) void k_bh_mass(BhMassArgs a) {
         o.x = a.lcx[s];
         o.y = a.lcy[s];
         o.z = a.lcz[s];
-        o.half = 0.0;
       } else {
         const double4 v = a.nmc[code];
         o.m = v.w;
         o.x = v.x;
         o.y = v.y;
         o.z = v.z;
-        o.half = a.nchild[(size_t)code * 8] == kBucket ? -1.0 : a.nbox[code].w;
+        o.dep = a.nchild[(size_t)code * 8] == kBucket ? -1 : a.d + 1;
       }
       sl[c] = o;
       mt += o.m;
@@ -450,7 +463,7 @@ __global__ __launch_bounds__(256) void k_bh_links(BhCounters *cnt, const int32_t
       if (!bh_live(sl[c])) continue;
       int first = -1;
       const int32_t code = sl[c].code;
-      if (code >= 0 && sl[c].half >= 0) {
+      if (code >= 0 && sl[c].dep >= 0) {
         const BhSlot *cs = slots + (size_t)code * 8;
         for (int c2 = 0; c2 < 8 && first < 0; c2++)
           if (bh_live(cs[c2])) first = code * 8 + c2;
@@ -487,8 +500,27 @@ struct BhForceArgs {
   int npad;
   int gsz;  // bodies per wave (lanes gsz..63 idle)
   double G, theta, soft2;
+  double theta2;  // RN(theta^2) when it is a normal number, else NaN (exact test always)
   int64_t node_cap;
 };
+
+// The reference's opening test RN(2 half / RN(sqrt(distSq))) < theta (octree.go:299-306),
+// decided from squares where it is clear-cut.  Let a = RN((2h)^2) and b = RN(RN(theta^2)
+// distSq), with RN(theta^2) normal (the host passes NaN otherwise) and b in [2^-1000, 2^1000].
+// Then each rounding (a when normal, theta^2, b, the margin product) and the square root's is
+// within 2^-53 relative, so a < b (1 - 2^-40) gives 2h / dist < theta (1 - 2^-43), below
+// theta's predecessor, and a > b (1 + 2^-40) gives 2h / dist > theta (1 + 2^-44), above its
+// successor: the rounded quotient lands on the same side of theta as the exact one.  a needs
+// no range check: below the normal range its absolute error (2^-1075) is 2^-74 of b, and an
+// infinite a exceeds any such b.  Anything else (within 2^-40 of the boundary, b out of range,
+// NaN) is left undecided for the reference's own expression.
+__device__ __forceinline__ void bh_quick_test(double h2, double distSq, double theta2,
+                                              bool &yes, bool &no) {
+  const double a = h2 * h2, b = theta2 * distSq;
+  const bool normal = b >= 0x1p-1000 && b <= 0x1p1000;
+  yes = normal && a < b * (1.0 - 0x1p-40);
+  no = normal && a > b * (1.0 + 0x1p-40);
+}
 
 // One interaction, op for op as accumForce / directForce (octree.go:292-298, 314-325).
 __device__ __forceinline__ void bh_term(double dx, double dy, double dz, double distSq,
@@ -500,9 +532,16 @@ __device__ __forceinline__ void bh_term(double dx, double dy, double dz, double 
   fz += force * dz / dist;
 }
 
+// The exact walk: one lane per body, bodies in key order.  Each step decides its slot (a leaf
+// is taken unless it is the body itself; a node by the opening test from squares, with the
+// reference's expression behind a wave-uniform branch only near the boundary), requests the
+// next slot and only then computes the taken slot's term, so the term's square root and
+// divisions run while the load is in flight.  Terms and their order are the reference's.
 __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
   const int M = a.cnt->M;
   if (M < 2) return;  // a lone body feels nothing; forces were zeroed
+  __shared__ double h2s[kBhLevels + 1];
+  bh_half2_table(a.nbox, h2s);
   const int lane = threadIdx.x & 63;
   const int s = (blockIdx.x * 4 + (int)(threadIdx.x >> 6)) * a.gsz + lane;
   bool act = lane < a.gsz && s < M;
@@ -516,7 +555,7 @@ __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
   unsigned long long nint = 0;
   if (act) {
     const double xi = a.sx[s], yi = a.sy[s], zi = a.sz[s], Gm = a.G * a.sm[s];
-    const double soft2 = a.soft2, theta = a.theta;
+    const double soft2 = a.soft2, theta = a.theta, theta2 = a.theta2;
     const double4 root = a.nmc[0];
     int q = -1;
     if (root.w != 0.0) {  // accumForce: n.totalMass == 0 returns
@@ -531,53 +570,58 @@ __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
         q = f < 0 ? -1 : f;
       }
     }
-    // One code path for leaves and nodes, so that lanes at different kinds of slot share the
-    // square root and the term: a leaf (the body's centre of mass (x*m)/m and mass 0 + m) is
-    // taken unless it is the body itself, a node when its opening test passes.
-    // The walk is pipelined by one slot: once the test has chosen the next slot, its 64 bytes
-    // are requested before the current term is computed, so the term's divisions run while the
-    // load is in flight.  The arithmetic and its order are unchanged.
-    int4 w0, w1, w2, w3;
+    int4 w0, w1, w2;
     auto fetch = [&](int slot) {
       const int4 *w = reinterpret_cast<const int4 *>(a.slots + slot);
       w0 = w[0];
       w1 = w[1];
       w2 = w[2];
-      w3 = w[3];
     };
-    if (q >= 0) fetch(q);
+    // Every step requests a slot, slot 0 (the root's first child, present when M >= 2) once the
+    // walk is done: an unconditional load lands straight in the registers the next step reads,
+    // where a conditional one is copied there and waited for before the term.
+    fetch(max(q, 0));
+    uint32_t nterm = 0;  // fewer than 2^32: one per visited slot
     while (q >= 0) {
       BhSlot sl;
       __builtin_memcpy(&sl, &w0, 16);
       __builtin_memcpy(reinterpret_cast<char *>(&sl) + 16, &w1, 16);
       __builtin_memcpy(reinterpret_cast<char *>(&sl) + 32, &w2, 16);
-      __builtin_memcpy(reinterpret_cast<char *>(&sl) + 48, &w3, 16);
       const bool leaf = sl.code <= -2;
-      const double dx = sl.x - xi, dy = sl.y - yi, dz = sl.z - zi;
-      if (!leaf && sl.half < 0) {  // depth limit: the bucket's bodies one by one (directForce)
+      if (!leaf && sl.dep < 0) {  // depth limit: the bucket's bodies one by one (directForce)
         const int32_t *ch = a.nchild + (size_t)sl.code * 8;
         const int b0 = ch[1], bn = ch[2];
         for (int s2 = b0; s2 < b0 + bn; s2++) {
           if (s2 == s) continue;
           const double ex = a.sx[s2] - xi, ey = a.sy[s2] - yi, ez = a.sz[s2] - zi;
-          const double distSq = ex * ex + ey * ey + ez * ez + soft2;
-          bh_term(ex, ey, ez, distSq, __builtin_sqrt(distSq), Gm, a.sm[s2], fx, fy, fz);
+          const double d2 = ex * ex + ey * ey + ez * ez + soft2;
+          bh_term(ex, ey, ez, d2, __builtin_sqrt(d2), Gm, a.sm[s2], fx, fy, fz);
           nint++;
         }
         q = sl.skip;
-        if (q >= 0) fetch(q);
+        fetch(max(q, 0));
         continue;
       }
+      const double dx = sl.x - xi, dy = sl.y - yi, dz = sl.z - zi;
       const double distSq = dx * dx + dy * dy + dz * dz + soft2;
-      const double dist = __builtin_sqrt(distSq);
-      const bool take = leaf ? (-2 - sl.code != s) : ((2 * sl.half / dist) < theta);
-      q = (leaf || take) ? sl.skip : sl.first;
-      if (q >= 0) fetch(q);
+      const double h2 = h2s[leaf ? 0 : sl.dep];
+      bool yes, no;
+      bh_quick_test(h2, distSq, theta2, yes, no);
+      const bool other = -2 - sl.code != s;
+      bool take = (leaf & other) | (!leaf & yes);
+      const bool und = !leaf & !yes & !no;
+      if (__builtin_amdgcn_ballot_w64(und)) {
+        __asm__ volatile("");  // a real branch: this division is rare
+        if (und) take = (h2 / __builtin_sqrt(distSq)) < theta;
+      }
+      q = (leaf | take) ? sl.skip : sl.first;
+      fetch(max(q, 0));
       if (take) {
-        bh_term(dx, dy, dz, distSq, dist, Gm, sl.m, fx, fy, fz);
-        nint++;
+        bh_term(dx, dy, dz, distSq, __builtin_sqrt(distSq), Gm, sl.m, fx, fy, fz);
+        nterm++;
       }
     }
+    nint += nterm;
     a.acc[i] = fx;
     a.acc[a.npad + i] = fy;
     a.acc[2 * a.npad + i] = fz;
@@ -685,8 +729,10 @@ __device__ __forceinline__ void fast_append(BhFastLds &L, bool has, float4 t, in
 
 __global__ __launch_bounds__(256) void k_bh_force_fast(BhForceArgs a) {
   __shared__ BhFastLds lds[4];
+  __shared__ double h2s[kBhLevels + 1];
   const int M = a.cnt->M;
   if (M < 2) return;
+  bh_half2_table(a.nbox, h2s);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   BhFastLds &L = lds[wv];
   const int s = (blockIdx.x * 4 + wv) * 64 + lane;
@@ -795,9 +841,9 @@ __global__ __launch_bounds__(256) void k_bh_force_fast(BhForceArgs a) {
           if (code <= -2) {  // a leaf: the body's own centre of mass (x*m)/m and mass 0 + m
             term = true;
             id = -2 - code;
-          } else if (sl.half < 0) {
+          } else if (sl.dep < 0) {
             bucket = code < node_lim;
-          } else if (accept(sl.x, sl.y, sl.z, sl.half)) {
+          } else if (accept(sl.x, sl.y, sl.z, 0.5 * h2s[sl.dep])) {
             term = true;
           } else {
             open = code < node_lim;
@@ -1161,15 +1207,18 @@ int bh_force_phase(Ctx *c, bool timed, int ev_slot) {
   fa.nmc = b->nmc;
   fa.nchild = b->nchild;
   fa.slots = b->slots;
-  // 64 bodies per wave unless asked: the walk is VALU-bound, and smaller groups only repeat
-  // the shared part of the walk in more waves (measured 6.9 / 8.4 / 10.7 ms at N = 65,536 for
-  // 64 / 32 / 16)
+  // 64 bodies per wave unless asked: smaller groups only repeat the shared part of the walk in
+  // more waves (measured 6.9 / 8.4 / 10.7 ms at N = 65,536 for 64 / 32 / 16)
   fa.gsz = b->p.group ? b->p.group : 64;
   fa.node_cap = b->node_cap;
   fa.acc = c->acc;
   fa.npad = (int)c->npad;
   fa.G = c->p.G;
   fa.theta = b->p.theta;
+  {
+    const double t2 = b->p.theta * b->p.theta;
+    fa.theta2 = (t2 >= 0x1p-1000 && t2 <= 0x1p1000) ? t2 : std::nan("");
+  }
   fa.soft2 = b->p.softening * b->p.softening;
   if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot], st));
   if (b->p.mode == 1)
