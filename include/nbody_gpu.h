@@ -272,7 +272,11 @@ typedef struct nb_bh_params {
                         to the group's bounding box), and the terms are summed in FP32 (the
                         body's own term excluded); forces then differ from exact mode by the
                         FP32 and opening differences */
-  int32_t reserved[5];
+  int32_t term;      /* exact mode's term: 0 = chosen by occupancy, 1 = three divisions by the
+                        distance, 2 = one shared reciprocal for them (the compiler's division
+                        steps, those that depend on the distance alone done once; bit-identical
+                        results, fewer instructions per term, a longer dependency chain) */
+  int32_t reserved[4];
 } nb_bh_params;
 
 typedef struct nb_bh_stats {

@@ -124,7 +124,8 @@ class NbSurStats(C.Structure):
 
 class NbBhParams(C.Structure):
     _fields_ = [("theta", C.c_double), ("softening", C.c_double), ("max_depth", C.c_int32),
-                ("group", C.c_int32), ("mode", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("group", C.c_int32), ("mode", C.c_int32), ("term", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class NbBhStats(C.Structure):

@@ -175,15 +175,17 @@ class NBodyGPU:
         return Bodies(*f, beh=beh, flags=flags, rest=rest, frag=frag, fstep=fstep)
 
     # -- Barnes-Hut octree (cmd/body/octree.go with Build wired in) ---------------------------
-    def bh_enable(self, theta=0.3, softening=1e-4, max_depth=21, group=0, mode="exact"):
+    def bh_enable(self, theta=0.3, softening=1e-4, max_depth=21, group=0, mode="exact", term=0):
         """The reference's octree force (FP64 contexts).  mode "exact": per-body walks with the
         reference's test and FP64 terms in its order; "fast": one walk per 64-body group that
-        opens what any of them would, FP32 terms."""
+        opens what any of them would, FP32 terms.  term (exact mode): 0 by occupancy, 1 three
+        divisions, 2 one shared reciprocal (bit-identical to 1)."""
         p = L.NbBhParams()
         check(lib.nb_bh_params_init(C.byref(p)))
         p.theta, p.softening, p.max_depth = float(theta), float(softening), int(max_depth)
         p.group = int(group)
         p.mode = {"exact": 0, "fast": 1}[mode] if isinstance(mode, str) else int(mode)
+        p.term = int(term)
         self._check(lib.nb_bh_enable(self._ctx, C.byref(p)))
 
     def bh_disable(self):

@@ -21,10 +21,10 @@ from nbodygo_amd import _lib as L
 pytestmark = pytest.mark.gpu
 
 
-def run(b, theta=0.3, max_depth=21, collisions=False):
+def run(b, theta=0.3, max_depth=21, collisions=False, term=0):
     with NBodyGPU(G=1.0, precision="fp64", collisions=collisions) as g:
         g.load(b)
-        g.bh_enable(theta=theta, max_depth=max_depth)
+        g.bh_enable(theta=theta, max_depth=max_depth, term=term)
         f = g.forces()
         return f, g.bh_depths(), g.bh_stats(), (g.pairs() if collisions else None)
 
@@ -41,10 +41,11 @@ def mixed(b, seed):
     return b
 
 
-def test_topology_and_theta_zero_are_exact(oracle):
+@pytest.mark.parametrize("term", [1, 2])  # three divisions; one shared reciprocal
+def test_topology_and_theta_zero_are_exact(oracle, term):
     b = mixed(gen.plummer(8000, seed=1), 1)
     st = oracle.State.from_bodies(b)
-    f, depth, stats, _ = run(b, theta=0.0)
+    f, depth, stats, _ = run(b, theta=0.0, term=term)
     fo, do, info = oracle.bh_force(st, G=1.0, theta=0.0)
     np.testing.assert_array_equal(depth, do)
     assert stats["nodes"] - stats["buckets"] == info["internal"]
@@ -70,6 +71,19 @@ def test_default_theta_matches_oracle(oracle, kind, n):
     # and it is a Barnes-Hut force: within the opening-angle error of the direct sum
     ref = oracle.force_soft(st, 1.0, 1e-4, idx=np.arange(0, b.n, 97), nthreads=16)
     assert np.linalg.norm(f[::97] - ref) / np.linalg.norm(ref) < 3e-3
+
+
+@pytest.mark.parametrize("n", [65536, 262144])
+def test_the_two_term_forms_are_bit_identical(n):
+    """The exact walk's term with three divisions and with one shared reciprocal give the same
+    bits at theta 0.3 (at 262,144 bodies the default picks the shared form, at 65,536 the
+    divisions)."""
+    b = gen.plummer(n, seed=21)
+    f1 = run(b, term=1)[0]
+    f2 = run(b, term=2)[0]
+    f0 = run(b)[0]
+    np.testing.assert_array_equal(f1, f2)
+    np.testing.assert_array_equal(f0, f1)
 
 
 def test_suns_are_direct_sums_bit_exact(oracle):
@@ -155,7 +169,7 @@ def test_rejected_configurations():
     with NBodyGPU(G=1.0, precision="fp64") as g:
         g.load(b)
         for bad in (dict(theta=-1.0), dict(max_depth=0), dict(max_depth=22),
-                    dict(softening=float("nan"))):
+                    dict(softening=float("nan")), dict(term=3), dict(term=-1)):
             with pytest.raises(NBodyError) as e:
                 g.bh_enable(**bad)
             assert e.value.code == L.NB_E_INVALID
