@@ -56,6 +56,8 @@ typedef const __attribute__((address_space(4))) f4v *cf4p;  // scalar-cache (uni
 typedef const __attribute__((address_space(4))) float *cfp;
 typedef const __attribute__((address_space(4))) int32_t *cip;
 
+constexpr int kNearSlots = 64;
+
 struct Sur {
   nb_sur_params p{};
   int G = 32, ncell = 0, K = 2;
@@ -90,7 +92,10 @@ struct Sur {
   double last_err = 0;
   float train_loss = 0;
   float *mu0 = nullptr, *sd1 = nullptr;  // identity normalisation (raw features)
-  double *near_pairs = nullptr;          // running count of exact near-field pairs
+  double *near_pairs = nullptr;          // running count of exact near-field pairs, in
+                                         // kNearSlots slots (workgroup w adds to slot w mod
+                                         // kNearSlots: one address for every workgroup of a
+                                         // launch serialises the atomics)
 };
 
 __device__ __forceinline__ bool act(uint8_t f) {
@@ -913,7 +918,9 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   }
   __syncthreads();
   if (wave != 0) return;
-  if (lane == 0) atomicAdd(a.near_pairs, (double)(t_end - t_begin) * (double)nsrc);
+  if (lane == 0)
+    atomicAdd(a.near_pairs + (blockIdx.x & (kNearSlots - 1)),
+              (double)(t_end - t_begin) * (double)nsrc);
   ax = ((ax + red[0][0][lane]) + red[1][0][lane]) + red[2][0][lane];
   ay = ((ay + red[0][1][lane]) + red[1][1][lane]) + red[2][1][lane];
   az = ((az + red[0][2][lane]) + red[1][2][lane]) + red[2][2][lane];
@@ -1528,7 +1535,7 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     NB_HIP(c, salloc(&s->sd1, kFeat));
     NB_HIP(c, salloc(&s->serr, 2));
     NB_HIP(c, salloc(&s->flag, 1));
-    NB_HIP(c, salloc(&s->near_pairs, 1));
+    NB_HIP(c, salloc(&s->near_pairs, kNearSlots));
     NB_HIP(c, salloc(&s->nitems, 1));
     std::vector<float> z(kFeat, 0.f), o(kFeat, 1.f);
     NB_HIP(c, hipMemcpy(s->mu0, z.data(), kFeat * 4, hipMemcpyHostToDevice));
@@ -1537,7 +1544,7 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     NB_HIP(c, hipMemcpy(s->sd, o.data(), kFeat * 4, hipMemcpyHostToDevice));
     NB_HIP(c, hipMemset(s->serr, 0, 2 * sizeof(double)));
     NB_HIP(c, hipMemset(s->flag, 0, sizeof(uint32_t)));
-    NB_HIP(c, hipMemset(s->near_pairs, 0, sizeof(double)));
+    NB_HIP(c, hipMemset(s->near_pairs, 0, kNearSlots * sizeof(double)));
   }
   // stretch radius: 1.5 x the half-mass radius about the centre of mass
   if (p->r0 > 0) {
@@ -1741,11 +1748,11 @@ int nb_sur_get_stats(nb_ctx *c, nb_sur_stats *out) {
   Sur *s = c->sur;
   NB_HIP(c, hipSetDevice(c->device));
   DevCounters ctr;
-  double err[2] = {0, 0}, pairs = 0;
+  double err[2] = {0, 0}, slots[kNearSlots];
   int32_t nec = 0;
   NB_HIP(c, hipMemcpyAsync(&ctr, c->ctr, sizeof ctr, hipMemcpyDeviceToHost, c->stream));
   NB_HIP(c, hipMemcpyAsync(err, s->serr, sizeof err, hipMemcpyDeviceToHost, c->stream));
-  NB_HIP(c, hipMemcpyAsync(&pairs, s->near_pairs, sizeof pairs, hipMemcpyDeviceToHost,
+  NB_HIP(c, hipMemcpyAsync(slots, s->near_pairs, sizeof slots, hipMemcpyDeviceToHost,
                            c->stream));
   NB_HIP(c, hipMemcpyAsync(&nec, s->necount, 4, hipMemcpyDeviceToHost, c->stream));
   NB_HIP(c, hipStreamSynchronize(c->stream));
@@ -1754,6 +1761,8 @@ int nb_sur_get_stats(nb_ctx *c, nb_sur_stats *out) {
   out->fallbacks = (int64_t)ctr.sur_fallbacks;
   out->last_error = err[0];
   out->last_body_error = err[1];
+  double pairs = 0;  // integer-valued terms below 2^53: the sum is exact in any order
+  for (int q = 0; q < kNearSlots; q++) pairs += slots[q];
   out->near_pairs_total = pairs;
   out->nonempty_cells = nec;
   out->train_rows = s->trows;
