@@ -730,11 +730,14 @@ __device__ void exact_chunk_pairs(const NearArgs &a, uint32_t i, int s0, int cnt
 // COLL: each lane also keeps the minimum r^2 of every chunk and compares it against a bound that
 // no overlapping pair can exceed in FP32 (the chunk's largest radius and coordinate, rounded up,
 // plus the FP32 rounding of positions); a flagged (body, chunk) is re-tested exactly in FP64.
+// LDS: one 3 KB region per wave, used in turn for the wave's source tile (64 float4), its staged
+// shell records (64 x 3 float4) and, in waves 1-3, its partial sums for wave 0 (3 x 64 doubles):
+// 12 KB per workgroup.  Without collisions the kernel is held to 64 VGPRs; with both, 8 waves
+// fit per SIMD.
 template <bool COLL>
-__global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
-  __shared__ double red[3][3][64];
-  __shared__ float4 tiles[4][64];
-  __shared__ float4 shrec[4][64][3];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COLL ? 1 : 8)))
+void k_sur_near(NearArgs a) {
+  __shared__ float4 wlds[4][64 * 3];
   const int item = blockIdx.x;
   if (item >= *a.nitems) return;  // uniform per workgroup
   const int G = a.G;
@@ -757,8 +760,8 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
   const bool tv = k < t_end;
   const float4 me = a.sp[tv ? k : t_begin];
   const float eps2 = to_vgpr(a.eps2);
-  float4 *tl = tiles[wave];
-  float4 (*rl)[3] = shrec[wave];
+  float4 *tl = wlds[wave];
+  float4 (*rl)[3] = reinterpret_cast<float4 (*)[3]>(wlds[wave]);
   double ax = 0, ay = 0, az = 0;
   int nsrc = 0;
   uint32_t i_body = 0;
@@ -912,18 +915,25 @@ __global__ __launch_bounds__(256) void k_sur_near(NearArgs a) {
     az += sz;
   }
   if (wave > 0) {
-    red[wave - 1][0][lane] = ax;
-    red[wave - 1][1][lane] = ay;
-    red[wave - 1][2][lane] = az;
+    // the wave's own region: its shell loop ended with a wave barrier after its last read
+    double (*rw)[64] = reinterpret_cast<double (*)[64]>(wlds[wave]);
+    rw[0][lane] = ax;
+    rw[1][lane] = ay;
+    rw[2][lane] = az;
   }
   __syncthreads();
   if (wave != 0) return;
   if (lane == 0)
     atomicAdd(a.near_pairs + (blockIdx.x & (kNearSlots - 1)),
               (double)(t_end - t_begin) * (double)nsrc);
-  ax = ((ax + red[0][0][lane]) + red[1][0][lane]) + red[2][0][lane];
-  ay = ((ay + red[0][1][lane]) + red[1][1][lane]) + red[2][1][lane];
-  az = ((az + red[0][2][lane]) + red[1][2][lane]) + red[2][2][lane];
+  {
+    double (*r1)[64] = reinterpret_cast<double (*)[64]>(wlds[1]);
+    double (*r2)[64] = reinterpret_cast<double (*)[64]>(wlds[2]);
+    double (*r3)[64] = reinterpret_cast<double (*)[64]>(wlds[3]);
+    ax = ((ax + r1[0][lane]) + r2[0][lane]) + r3[0][lane];
+    ay = ((ay + r1[1][lane]) + r2[1][lane]) + r3[1][lane];
+    az = ((az + r1[2][lane]) + r2[2][lane]) + r3[2][lane];
+  }
   for (int o = cwid; o < 64; o <<= 1) {  // the copies, pairwise
     ax += __shfl_xor(ax, o);
     ay += __shfl_xor(ay, o);
