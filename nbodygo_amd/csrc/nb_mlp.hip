@@ -27,10 +27,9 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
-  // round to nearest even (inputs are finite)
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  // round to nearest even: one v_cvt_pk_bf16_f32 (for finite inputs the same bits as adding
+  // 0x7fff + the kept lowest bit to the FP32 word and taking its upper half)
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -41,10 +40,15 @@ __device__ __forceinline__ float tanh_fast(float u) {
   return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
 }
 
-// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
+// GELU, tanh form: 0.5 x (1 + tanh(z)), z = sqrt(2/pi) (x + 0.044715 x^3).  Since
+// 1 + tanh(z) = 2 - 2 / (e^(2z) + 1), this is x - x / (e^(2z) + 1), and e^(2z) = 2^(x (c0 + c1 x^2))
+// with c0 = 2 log2(e) sqrt(2/pi), c1 = 0.044715 c0: five VALU operations around the hardware
+// exponential and reciprocal (the surrogate's inference evaluates it 96 times per lane and tile).
+// Large x gives x, very negative x gives 0.
 __device__ __forceinline__ float gelu(float x) {
-  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-  return 0.5f * x * (1.f + tanh_fast(k0 * (x + k1 * x * x * x)));
+  const float c0 = 2.302208198144325f, c1 = 0.1029432395800235f;
+  const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(c1, x * x, c0));
+  return __builtin_fmaf(-x, __builtin_amdgcn_rcpf(e + 1.f), x);
 }
 __device__ __forceinline__ float gelu_grad(float x) {
   const float k0 = 0.7978845608028654f, k1 = 0.044715f;
@@ -761,7 +765,18 @@ int nb_mlp_create(int32_t device, int32_t din, int32_t hidden, int32_t layers, i
       layers > 8 || dout < 1 || dout > 64)
     return NB_E_INVALID;
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return NB_E_HIP;
+  // a HIP failure is reported with its error string through nb_last_error(NULL)
+  auto hip_fail = [](const char *what, hipError_t e) {
+    static thread_local std::string msg;
+    msg = std::string("nb_mlp_create: ") + what + ": " + hipGetErrorString(e);
+    set_global_error(msg.c_str());
+    return NB_E_HIP;
+  };
+  {
+    const hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess) return hip_fail("hipGetDeviceCount", e);
+    if (device < 0 || device >= ndev) return hip_fail("device", hipErrorInvalidDevice);
+  }
   nb_mlp *m = new (std::nothrow) nb_mlp();
   if (!m) return NB_E_NOMEM;
   m->device = device;
@@ -771,10 +786,13 @@ int nb_mlp_create(int32_t device, int32_t din, int32_t hidden, int32_t layers, i
   m->layers = layers;
   m->dout = dout;
   m->doutp = 64;
-  if (hipSetDevice(device) != hipSuccess ||
-      hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete m;
-    return NB_E_HIP;
+  {
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete m;
+      return hip_fail("stream", e);
+    }
   }
   std::mt19937_64 rng(seed);
   for (int l = 0; l <= layers; l++) {
@@ -826,9 +844,13 @@ int nb_mlp_create(int32_t device, int32_t din, int32_t hidden, int32_t layers, i
     hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk((int64_t)nw)), dim3(256), 0, m->stream, W, Wb,
                        (int64_t)nw);
   }
-  if (hipStreamSynchronize(m->stream) != hipSuccess) {
-    nb_mlp_destroy(m);
-    return NB_E_HIP;
+  {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) {
+      nb_mlp_destroy(m);
+      return hip_fail("weight conversion", e);
+    }
   }
   *out = m;
   return NB_OK;

@@ -16,7 +16,7 @@ class MLP:
         self._m = C.c_void_p()
         rc = lib.nb_mlp_create(device, d_in, hidden, layers, d_out, seed, C.byref(self._m))
         if rc != L.NB_OK:
-            raise L.NBodyError(rc, "nb_mlp_create failed")
+            raise L.NBodyError(rc, "nb_mlp_create failed: " + lib.nb_last_error(None).decode())
         self.d_in, self.hidden, self.layers, self.d_out = d_in, hidden, layers, d_out
         n = C.c_int64()
         self._check(lib.nb_mlp_param_count(self._m, C.byref(n)))
