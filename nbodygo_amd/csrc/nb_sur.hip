@@ -103,11 +103,17 @@ __device__ __forceinline__ bool act(uint8_t f) {
 }
 
 // ---- centre of mass and largest radius (two-stage, fixed order) ------------------------------
+// zero_ctr (may be null): the cycle's pair and record counts are reset here, the first kernel of
+// the force phase, instead of by a launch of their own.
 __global__ __launch_bounds__(256) void k_sur_com_part(const double *x, const double *y,
                                                       const double *z, const double *m,
                                                       const double *r, const uint8_t *flags,
-                                                      int n, double *part) {
+                                                      int n, double *part, DevCounters *zero_ctr) {
   __shared__ double sh[5][256];
+  if (zero_ctr && blockIdx.x == 0 && threadIdx.x == 0) {
+    zero_ctr->pair_count = 0;
+    zero_ctr->rec_count = 0;
+  }
   double s[5] = {0, 0, 0, 0, 0};
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
     if (act(flags[i])) {
@@ -1183,9 +1189,14 @@ __global__ __launch_bounds__(256) void k_sur_check(const int32_t *sidx, int S,
 }
 
 // After a fallback force launch: fold the plain kernel's slices into slice 0 (flag set only).
-__global__ void k_sur_fold(double *acc, int split, int npad, int n, const uint32_t *flag) {
+// After a fallback (flag up): the direct sum's slices folded into slice 0, and the fallback
+// counted once.
+__global__ void k_sur_fold(double *acc, int split, int npad, int n, const uint32_t *flag,
+                           DevCounters *ctr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !*flag) return;
+  if (!*flag) return;
+  if (i == 0) atomicAdd((unsigned long long *)&ctr->sur_fallbacks, 1ull);
+  if (i >= n) return;
   for (int c = 0; c < 3; c++) {
     double s = 0;
     for (int q = 0; q < split; q++) s += acc[((size_t)q * 3 + c) * npad + i];
@@ -1257,10 +1268,6 @@ __global__ void k_sur_samples(int32_t *sidx, int S, const int32_t *cstart, int n
   int64_t k = (int64_t)(((double)s + u) * (double)nact / (double)S);
   if (k >= nact) k = nact - 1;
   sidx[s] = nact > 0 ? (int)k : 0;
-}
-
-__global__ void k_sur_count_fallback(const uint32_t *flag, DevCounters *ctr) {
-  if (threadIdx.x == 0 && *flag) atomicAdd((unsigned long long *)&ctr->sur_fallbacks, 1ull);
 }
 
 template <class T>
@@ -1360,7 +1367,7 @@ static int sur_check_launch(Ctx *c, int S, double tol, double tol_body, bool rot
 // Binning, moments, far expansion, near field (+ features).  apply: scatter into acc slice 0.
 // detect: collision pairs from the near field into the context's pair list.
 static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool use_mlp,
-                        bool detect = false) {
+                        bool detect = false, bool zero_counters = false) {
   Sur *s = c->sur;
   hipStream_t st = c->stream;
   const int n = (int)c->n, G = s->G, ncell = s->ncell;
@@ -1369,7 +1376,7 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
   if (rc) return rc;
   const int nbk = (int)std::min<int64_t>(1024, (n + 255) / 256);
   hipLaunchKernelGGL(k_sur_com_part, dim3(nbk), dim3(256), 0, st, c->x, c->y, c->z, c->m, c->r,
-                     c->flags, n, s->red);
+                     c->flags, n, s->red, zero_counters ? c->ctr : nullptr);
   hipLaunchKernelGGL(k_sur_com_final, dim3(1), dim3(256), 0, st, s->red, nbk, s->center);
   hipLaunchKernelGGL(k_sur_keys, dim3(nb256(n)), dim3(256), 0, st, c->x, c->y, c->z, c->flags,
                      n, s->center, s->r0, G, s->keys, s->perm_in);
@@ -1446,17 +1453,17 @@ int sur_force_phase(Ctx *c, bool timed, int ev_slot) {
   if (c->p.collisions && !(c->rmax <= s->r0 / s->G * (1.0 - 1e-6)))  // a later nb_load
     return fail_msg(c, NB_E_UNSUPPORTED,
                     "collisions with the surrogate need every radius <= r0 / grid");
-  NB_HIP(c, launch_zero_counters(c->ctr, 0, st));
+  // the pipeline's first kernel resets the cycle's counters (it launches nothing without bodies)
+  if (c->n == 0) NB_HIP(c, launch_zero_counters(c->ctr, 0, st));
   if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot], st));
   const bool detect = c->p.collisions != 0;
-  int rc = sur_pipeline(c, false, s->p.use_mlp && s->trained, true, s->p.use_mlp != 0, detect);
+  int rc = sur_pipeline(c, false, s->p.use_mlp && s->trained, true, s->p.use_mlp != 0, detect,
+                        true);
   if (rc) return rc;
   if (timed) NB_HIP(c, hipEventRecord(c->ev[2 * ev_slot + 1], st));
   if (s->p.check_every > 0 && s->steps % s->p.check_every == 0 && c->n > 0) {
     rc = sur_check_launch(c, s->p.check_samples, s->p.tol, s->p.tol_body, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sur_count_fallback, dim3(1), dim3(64), 0, st, s->flag, c->ctr);
-    NB_HIP(c, hipGetLastError());
     // direct-sum redo of this cycle's force when the flag is up (the kernel exits otherwise)
     ForceArgs fa = make_force_args(c);
     fa.run_flag = s->flag;
@@ -1464,7 +1471,7 @@ int sur_force_phase(Ctx *c, bool timed, int ev_slot) {
     fa.drop_flagged = 0;
     NB_HIP(c, launch_force_f32(c->shape, fa, st));
     hipLaunchKernelGGL(k_sur_fold, dim3(nb256(c->n)), dim3(256), 0, st, c->acc, c->shape.jsplit,
-                       (int)c->npad, (int)c->n, s->flag);
+                       (int)c->npad, (int)c->n, s->flag, c->ctr);
     NB_HIP(c, hipGetLastError());
     s->checks++;
   }
