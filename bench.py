@@ -343,6 +343,8 @@ def collide_leg(a, pinned, steps):
             out[mode]["error_without_mlp"] = err_base
             out[mode]["checks"] = s1["checks"] - s0["checks"]
             out[mode]["fallbacks"] = s1["fallbacks"] - s0["fallbacks"]
+            # exact near-field pairs per cycle (the near kernel's work; the checks add none)
+            out[mode]["near_pairs_per_step"] = (s1["near_pairs_total"] - s0["near_pairs_total"]) / steps
     g.close()
     fs, fp = out["surrogate"].pop("final"), out["plain"].pop("final")
     dpos = float(np.sqrt(np.mean(np.sum((fs.pos() - fp.pos()) ** 2, axis=1))))
