@@ -153,9 +153,12 @@ def measured_traffic(n, shape):
             t = json.load(f)
     except (OSError, ValueError):
         return None, None
-    if t.get("n") != n or t.get("shape") != shape:
-        return None, "profiles/" + name + " (taken at another shape: null)"
-    return t["bytes_per_launch"], "profiles/" + name
+    if t.get("n") != n:
+        return None, "profiles/" + name + " (taken at another body count: null)"
+    for e in t.get("shapes", [t]):  # one entry per measured shape (or a single measurement)
+        if e.get("shape") == shape:
+            return e["bytes_per_launch"], "profiles/" + name
+    return None, "profiles/" + name + " (taken at other shapes: null)"
 
 
 def timed_run(g, b, steps, warmup, dt=1e-3):
