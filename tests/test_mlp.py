@@ -47,46 +47,6 @@ def test_gelu_derivative_and_bf16_rounding():
     assert R.bf16(np.float32(3.0)) == 3.0
 
 
-def test_oracle_forward_and_gradient_match_torch():
-    """mlp_ref against an independent statement of the same network: torch's tanh-form GELU,
-    its linear layers and autograd, in float64 (no bf16 rounding), so a wrong GELU constant, a
-    dropped bias or a transposed weight in the oracle fails here."""
-    torch = pytest.importorskip("torch")
-    F = torch.nn.functional
-    params = random_params(5, d_in=6, h=16, layers=3, d_out=3)
-    rng = np.random.default_rng(6)
-    x = rng.normal(size=(33, 6))
-    t = rng.normal(size=(33, 3))
-    ws = [(torch.tensor(W, requires_grad=True), torch.tensor(b, requires_grad=True))
-          for W, b in params]
-    h = torch.tensor(x)
-    for W, b in ws[:-1]:
-        h = F.gelu(F.linear(h, W.T, b), approximate="tanh")
-    W, b = ws[-1]
-    y = F.linear(h, W.T, b)
-    loss = torch.mean((y - torch.tensor(t)) ** 2)
-    loss.backward()
-    y_or, _ = R.forward(params, x)
-    np.testing.assert_allclose(y_or, y.detach().numpy(), rtol=1e-12, atol=1e-12)
-    l_or, g_or = R.loss_and_grad(params, x, t)
-    assert abs(l_or - loss.item()) <= 1e-12 * abs(loss.item())
-    g_t = R.pack([(W.grad.numpy(), b.grad.numpy()) for W, b in ws])
-    np.testing.assert_allclose(g_or, g_t, rtol=1e-10, atol=1e-13)
-    z = np.linspace(-8, 8, 2001)
-    np.testing.assert_allclose(R.gelu(z), F.gelu(torch.tensor(z), approximate="tanh").numpy(),
-                               rtol=1e-13, atol=1e-15)
-
-
-def test_bf16_rounding_matches_torch():
-    torch = pytest.importorskip("torch")
-    rng = np.random.default_rng(7)
-    v = np.concatenate([rng.normal(size=20000) * 10.0 ** rng.integers(-20, 20, 20000),
-                        np.float32(1.0) + np.arange(-512, 512) * np.float32(2.0 ** -16)])
-    v32 = v.astype(np.float32)
-    ref = torch.tensor(v32).to(torch.bfloat16).to(torch.float64).numpy()
-    np.testing.assert_array_equal(R.bf16(v32), ref)
-
-
 @pytest.mark.gpu
 def test_device_forward_matches_oracle():
     from nbodygo_amd.mlp import MLP
@@ -161,3 +121,47 @@ def test_device_training_reduces_loss():
         y1 = m.forward(x[:4096])
         l1 = float(np.mean((y1 - t[:4096]) ** 2))
     assert l1 < l0 / 10 and lt < l0 / 10
+
+
+# The two tests below run torch on the CPU.  They come after this file's GPU tests: in a
+# process whose first GPU use comes after torch's autograd has started, hipGetDeviceCount has
+# been seen to report no device on the GPU boxes, so a run of this file alone keeps the GPU
+# tests first.
+def test_oracle_forward_and_gradient_match_torch():
+    """mlp_ref against an independent statement of the same network: torch's tanh-form GELU,
+    its linear layers and autograd, in float64 (no bf16 rounding), so a wrong GELU constant, a
+    dropped bias or a transposed weight in the oracle fails here."""
+    torch = pytest.importorskip("torch")
+    F = torch.nn.functional
+    params = random_params(5, d_in=6, h=16, layers=3, d_out=3)
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(33, 6))
+    t = rng.normal(size=(33, 3))
+    ws = [(torch.tensor(W, requires_grad=True), torch.tensor(b, requires_grad=True))
+          for W, b in params]
+    h = torch.tensor(x)
+    for W, b in ws[:-1]:
+        h = F.gelu(F.linear(h, W.T, b), approximate="tanh")
+    W, b = ws[-1]
+    y = F.linear(h, W.T, b)
+    loss = torch.mean((y - torch.tensor(t)) ** 2)
+    loss.backward()
+    y_or, _ = R.forward(params, x)
+    np.testing.assert_allclose(y_or, y.detach().numpy(), rtol=1e-12, atol=1e-12)
+    l_or, g_or = R.loss_and_grad(params, x, t)
+    assert abs(l_or - loss.item()) <= 1e-12 * abs(loss.item())
+    g_t = R.pack([(W.grad.numpy(), b.grad.numpy()) for W, b in ws])
+    np.testing.assert_allclose(g_or, g_t, rtol=1e-10, atol=1e-13)
+    z = np.linspace(-8, 8, 2001)
+    np.testing.assert_allclose(R.gelu(z), F.gelu(torch.tensor(z), approximate="tanh").numpy(),
+                               rtol=1e-13, atol=1e-15)
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(7)
+    v = np.concatenate([rng.normal(size=20000) * 10.0 ** rng.integers(-20, 20, 20000),
+                        np.float32(1.0) + np.arange(-512, 512) * np.float32(2.0 ** -16)])
+    v32 = v.astype(np.float32)
+    ref = torch.tensor(v32).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(R.bf16(v32), ref)
