@@ -92,6 +92,9 @@ struct Sur {
   double last_err = 0;
   float train_loss = 0;
   float *mu0 = nullptr, *sd1 = nullptr;  // identity normalisation (raw features)
+  // the item count of the last cycle whose scan has run, written by k_sur_scan into pinned host
+  // memory: the near kernel's grid (its workgroups loop over items, so any size is correct)
+  int32_t *nitems_host = nullptr, *nitems_host_dev = nullptr;
   double *near_pairs = nullptr;          // running count of exact near-field pairs, in
                                          // kNearSlots slots (workgroup w adds to slot w mod
                                          // kNearSlots: one address for every workgroup of a
@@ -254,7 +257,7 @@ __global__ __launch_bounds__(1024) void k_sur_bounds(const uint32_t *keys2, int 
 __global__ __launch_bounds__(1024) void k_sur_scan(const int32_t *cstart, int ncell,
                                                    const int2 *btot, int32_t *ccount,
                                                    int32_t *nelist, int32_t *necount, int2 *items,
-                                                   int32_t *nitems) {
+                                                   int32_t *nitems, int32_t *nitems_host) {
   __shared__ int32_t wa[16], wb[16], base[2];
   const int c = blockIdx.x * 1024 + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (threadIdx.x == 0) {
@@ -268,6 +271,7 @@ __global__ __launch_bounds__(1024) void k_sur_scan(const int32_t *cstart, int nc
     if (blockIdx.x == gridDim.x - 1) {
       *necount = sa + btot[blockIdx.x].x;
       *nitems = sb + btot[blockIdx.x].y;
+      if (nitems_host) *nitems_host = sb + btot[blockIdx.x].y;  // the next cycle's grid hint
     }
   }
   const int cnt = c < ncell ? cstart[c + 1] - cstart[c] : 0;
@@ -741,11 +745,8 @@ __device__ void exact_chunk_pairs(const NearArgs &a, uint32_t i, int s0, int cnt
 // 12 KB per workgroup.  Without collisions the kernel is held to 64 VGPRs; with both, 8 waves
 // fit per SIMD.
 template <bool COLL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COLL ? 1 : 8)))
-void k_sur_near(NearArgs a) {
-  __shared__ float4 wlds[4][64 * 3];
-  const int item = blockIdx.x;
-  if (item >= *a.nitems) return;  // uniform per workgroup
+__device__ __forceinline__ void near_item(const NearArgs &a, const int item,
+                                          float4 (*wlds)[64 * 3]) {
   const int G = a.G;
   const int2 it = a.items[item];
   const int c = it.x;
@@ -1017,6 +1018,20 @@ void k_sur_near(NearArgs a) {
   }
 }
 
+// One workgroup per item (the launch may be smaller than the item count: workgroups then take
+// further items; the barrier between two items keeps wave 0's reads of the partial sums ahead of
+// the other waves' next writes).
+template <bool COLL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COLL ? 1 : 8)))
+void k_sur_near(NearArgs a) {
+  __shared__ float4 wlds[4][64 * 3];
+  const int nitems = *a.nitems;
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    if (item != (int)blockIdx.x) __syncthreads();
+    near_item<COLL>(a, item, wlds);
+  }
+}
+
 // Scatter the surrogate's accelerations (base + learned residual) to the accumulator slice 0.
 __global__ void k_sur_apply(const uint32_t *perm, const uint32_t *keys2, int n, int ncell,
                             const float4 *base, const float *Y, int ldy, float out_scale,
@@ -1281,6 +1296,7 @@ static hipError_t salloc(T **p, size_t count) {
 
 void sur_free(Sur *s) {
   if (!s) return;
+  if (s->nitems_host) (void)hipHostFree(s->nitems_host);
   void *ptrs[] = {s->keys, s->keys2, s->perm_in, s->perm, s->sort_tmp, s->cstart, s->ccount,
                   s->nelist, s->necount, s->sp, s->crec, s->crec_ne, s->fexp, s->part,
                   s->items, s->nitems, s->btot, s->feat, s->base, s->center,
@@ -1388,7 +1404,7 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
                      s->keys2, n, ncell, s->cstart, s->btot);
   hipLaunchKernelGGL(k_sur_scan, dim3((unsigned)((ncell + 1023) / 1024)), dim3(1024), 0, st,
                      s->cstart, ncell, s->btot, s->ccount, s->nelist, s->necount, s->items,
-                     s->nitems);
+                     s->nitems, s->nitems_host_dev);
   hipLaunchKernelGGL(k_sur_gather, dim3(nb256(n)), dim3(256), 0, st, s->perm, s->keys2, n, ncell,
                      c->x, c->y, c->z, c->m, s->center, s->sp);
   hipLaunchKernelGGL(k_sur_moments, dim3((unsigned)ncell), dim3(64), 0, st, s->sp, s->cstart,
@@ -1428,11 +1444,16 @@ static int sur_pipeline(Ctx *c, bool feat_raw, bool feat_norm, bool apply, bool 
   na.ctr = c->ctr;
   na.pair_cap = c->pair_cap;
   na.events = c->p.events;
-  const unsigned item_ub = (unsigned)(ncell + (n + 63) / 64 + 1);
+  // one workgroup per item: the bound on items, or a quarter more than the last count seen
+  // (workgroups loop over items, so a count that has grown since only costs a second trip)
+  const int64_t item_ub = (int64_t)ncell + (n + 63) / 64 + 1;
+  const int32_t seen = *(volatile int32_t *)s->nitems_host;
+  const unsigned grid =
+      (unsigned)(seen > 0 ? std::min<int64_t>(item_ub, (int64_t)seen + seen / 4 + 256) : item_ub);
   if (detect)
-    hipLaunchKernelGGL(k_sur_near<true>, dim3(item_ub), dim3(256), 0, st, na);
+    hipLaunchKernelGGL(k_sur_near<true>, dim3(grid), dim3(256), 0, st, na);
   else
-    hipLaunchKernelGGL(k_sur_near<false>, dim3(item_ub), dim3(256), 0, st, na);
+    hipLaunchKernelGGL(k_sur_near<false>, dim3(grid), dim3(256), 0, st, na);
   NB_HIP(c, hipGetLastError());
   if (!apply) return NB_OK;
   const bool mlp_on = use_mlp && s->trained && s->mlp;
@@ -1524,6 +1545,9 @@ int nb_sur_enable(nb_ctx *c, const nb_sur_params *p) {
     s = new (std::nothrow) Sur();
     if (!s) return fail_msg(c, NB_E_NOMEM, "out of host memory");
     c->sur = s;
+    NB_HIP(c, hipHostMalloc((void **)&s->nitems_host, sizeof(int32_t), hipHostMallocMapped));
+    *s->nitems_host = -1;
+    NB_HIP(c, hipHostGetDevicePointer((void **)&s->nitems_host_dev, s->nitems_host, 0));
   }
   const bool grid_changed = fresh || s->G != p->grid;
   s->p = *p;

@@ -112,6 +112,25 @@ def test_check_passes_under_loose_tolerance_and_is_deterministic():
         np.testing.assert_array_equal(getattr(out[0], k), getattr(out[1], k))
 
 
+def test_near_field_workgroups_take_several_items_when_the_count_grows():
+    """The near kernel's grid follows the previous cycle's item count (plus a margin); a state
+    with many more items than the last one makes workgroups take several items each, and the
+    forces must be the bits of a fresh context, whose first launch has a workgroup per item."""
+    small, big = gen.plummer(2048, seed=31), gen.plummer(131072, seed=32)
+    r0 = 1.5  # the same stretched grid for both contexts
+    with sim() as g:
+        g.load(small)
+        g.sur_enable(r0=r0, use_mlp=False, check_every=0)
+        g.forces()
+        g.load(big)
+        f_reused = g.forces()
+    with sim() as h:
+        h.load(big)
+        h.sur_enable(r0=r0, use_mlp=False, check_every=0)
+        f_fresh = h.forces()
+    np.testing.assert_array_equal(f_reused, f_fresh)
+
+
 def test_unsupported_configurations_are_rejected():
     b = gen.plummer(2048, seed=8)
     for kw, code in ((dict(G=1.0, eps=EPS, collisions=True), L.NB_E_UNSUPPORTED),  # the gate
