@@ -65,8 +65,8 @@ struct BhCounters {
 // opened) and `skip` (slot visited next once this subtree is done; -1 ends the walk), and `dep`:
 // a node's depth (its half is the root's halved dep times, exactly as the build halves it), -1
 // for a bucket, 0 for a leaf or an empty slot.  A slot is live when its code is not -1 and its
-// mass is > 0.  48 bytes: three 16-byte loads per visit (the walk's reads of the slots bound
-// it), the half coming from a per-depth table.
+// mass is > 0.  48 bytes: three 16-byte loads per visit instead of four, the half coming from
+// a per-depth table.
 struct alignas(16) BhSlot {
   double x, y, z, m;
   int32_t code, first, skip, dep;
@@ -622,9 +622,9 @@ __global__ __launch_bounds__(256) void k_bh_force(BhForceArgs a) {
       w1 = w[1];
       w2 = w[2];
     };
-    // Every step requests a slot, slot 0 (the root's first child, present when M >= 2) once the
-    // walk is done: an unconditional load lands straight in the registers the next step reads,
-    // where a conditional one is copied there and waited for before the term.
+    // Every step requests a slot, slot 0 (node 0's first slot, always allocated) once the walk is
+    // done: an unconditional load lands straight in the registers the next step reads, where a
+    // conditional one is copied there and waited for before the term.
     fetch(max(q, 0));
     uint32_t nterm = 0;  // fewer than 2^32: one per visited slot
     while (q >= 0) {
