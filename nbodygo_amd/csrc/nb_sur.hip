@@ -797,6 +797,25 @@ __device__ __forceinline__ void near_item(const NearArgs &a, const int item,
     row_range(q, b, l);
     nsrc += l;
   }
+  // multipole shell positions (below): the first round's cell counts are requested here, so that
+  // their latency overlaps the first source chunk's
+  const int K = a.K, D = 2 * K + 1, box = D * D * D;
+  const float invD = 1.f / (float)D;
+  auto shell_pos = [&](int p, int &cell) -> bool {
+    if (p >= box) return false;
+    const int pq = (int)(((float)p + 0.5f) * invD);     // p / D
+    const int pr = (int)(((float)pq + 0.5f) * invD);    // p / D^2
+    const int dx = p - pq * D - K, dy = pq - pr * D - K, dz = pr - K;
+    const int x = cx + dx, y = cy + dy, z = cz + dz;
+    if (max(abs(dx), max(abs(dy), abs(dz))) < 2 || x < 0 || x >= G || y < 0 || y >= G || z < 0 ||
+        z >= G)
+      return false;
+    cell = x + G * (y + G * z);
+    return true;
+  };
+  int cell0 = 0;
+  const bool in0 = shell_pos(4 * lane + wave, cell0);
+  const int cnt0 = in0 ? a.ccount[cell0] : 0;
   const int share = (nsrc + 3) / 4;
   const int w0 = wave * share, w1 = min(nsrc, w0 + share);
   int base = 0;
@@ -884,23 +903,13 @@ __device__ __forceinline__ void near_item(const NearArgs &a, const int item,
   // for p < (2K+1)^3 <= 2197 the product (p + 0.5) / D lies at least 0.5 / D from an integer,
   // far beyond its rounding, so the truncation is the exact quotient.
   {
-    const int K = a.K, D = 2 * K + 1, box = D * D * D;
-    const float invD = 1.f / (float)D;
     float sx = 0, sy = 0, sz = 0;
     for (int p0 = 0; p0 < box; p0 += 256) {
-      const int p = p0 + 4 * lane + wave;
-      bool ok = false;
-      int cell = 0;
-      if (p < box) {
-        const int pq = (int)(((float)p + 0.5f) * invD);     // p / D
-        const int pr = (int)(((float)pq + 0.5f) * invD);    // p / D^2
-        const int dx = p - pq * D - K, dy = pq - pr * D - K, dz = pr - K;
-        const int x = cx + dx, y = cy + dy, z = cz + dz;
-        if (max(abs(dx), max(abs(dy), abs(dz))) >= 2 && x >= 0 && x < G && y >= 0 && y < G &&
-            z >= 0 && z < G) {
-          cell = x + G * (y + G * z);
-          ok = a.ccount[cell] > 0;
-        }
+      int cell = cell0;
+      bool ok = cnt0 > 0;
+      if (p0 > 0) {
+        ok = false;
+        if (shell_pos(p0 + 4 * lane + wave, cell)) ok = a.ccount[cell] > 0;
       }
       const uint64_t okm = __ballot(ok);
       const int slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(okm >> 32),
