@@ -152,6 +152,28 @@ def test_fp32_plummer_force_error_config2(oracle):
     assert m64 < 1e-13
 
 
+@pytest.mark.parametrize("shape", [(1, 512, 4), (1, 256, 4), (2, 256, 8)])
+def test_headline_force_error_at_size(oracle, shape):
+    """The headline workload (N = 262,144 Plummer, eps = 0.01) in each launch shape the autotuner
+    picks at that size (targets per lane, source tile, source slices): force error vs the FP64
+    oracle on 1,024 sampled targets, to the config-2 bars."""
+    n = 262144
+    b = gen.plummer(n, seed=2024)
+    st = oracle.State.from_bodies(b)
+    idx = np.random.default_rng(1).choice(n, 1024, replace=False)
+    exp = oracle.force_soft(st, 1.0, 0.01, idx=idx, nthreads=NT)
+    ipl, tile_j, jsplit = shape
+    with NBodyGPU(G=1.0, eps=0.01, overlap_cutoff=False, collisions=False, ipl=ipl,
+                  tile_j=tile_j, jsplit=jsplit, block=256, variant=2) as g:
+        g.load(b)
+        f = g.forces()[idx]
+    glob, p50, p99, mx = rel_err(f, exp)
+    print("N=262144 shape %s FP32 force error: global %.3e p99 %.3e max %.3e" % (shape, glob,
+                                                                                 p99, mx))
+    assert glob <= 1e-5
+    assert p99 <= 1e-4
+
+
 def test_fp32_reference_mode_force_error(oracle):
     """Config 1 in reference semantics (eps = 0, overlap gate): the gate decisions are exact
     FP64, so every body's force is within FP32 error of the oracle's."""
