@@ -1,16 +1,18 @@
 // nb_mlp.hip -- small MLP (d_in -> H -> ... -> H -> d_out, GELU) on gfx950 matrix cores.
 //
 // No counterpart in the reference.  It is the learned part of the far-field surrogate: BF16
-// weights and activations feed v_mfma_f32_16x16x16_bf16 with FP32 accumulation; master weights,
-// biases, pre-activations, gradients and the Adam state are FP32.
+// weights and activations feed the matrix cores with FP32 accumulation (gfx950's
+// v_mfma_f32_16x16x32_bf16 in the forward pass and inference, v_mfma_f32_16x16x16_bf16 in the
+// backward GEMMs); master weights, biases, pre-activations, gradients and the Adam state are FP32.
 //
 // GEMM: C[M x N] (FP32, row-major) = op(A)[M x K] * op(B)[K x N], A and B BF16 row-major, op =
 // identity or transpose.  A 256-thread workgroup computes a 64 x 64 tile as 2 x 2 waves of 32 x 32,
-// each wave 2 x 2 MFMA blocks of 16 x 16, stepping K by 16 through LDS.  Operand layout of the 16
-// x 16 x 16 BF16 MFMA (checked against a host GEMM in tests): lane l holds A[l % 16][4*(l/16) ..
-// +3] and B[4*(l/16) .. +3][l % 16]; the accumulator holds D[4*(l/16) + v][l % 16] in register v.
-// Split-K over grid.z writes partial tiles that a second kernel sums in fixed order.
-// Dimensions are padded by the caller: M, N multiples of 64, K a multiple of 16.
+// each wave 2 x 2 MFMA blocks of 16 x 16, stepping K by 16 (or 32) through LDS.  Operand layout of
+// the 16 x 16 x 16 BF16 MFMA (checked against a host GEMM in tests): lane l holds A[l % 16][4*(l/16)
+// .. +3] and B[4*(l/16) .. +3][l % 16]; the 16 x 16 x 32 form holds eight, 8*(l/16) .. +7; the
+// accumulator holds D[4*(l/16) + v][l % 16] in register v.  Split-K over grid.z writes partial
+// tiles that a second kernel sums in fixed order.  Dimensions are padded by the caller: M, N
+// multiples of 64, K a multiple of 16.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -24,6 +26,7 @@
 namespace nb {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));  // gfx950's 16 x 16 x 32 MFMA operands
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -59,15 +62,20 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 enum { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2 };
 
-template <bool TA, bool TB>
+// KS = 32 (no transposes; K and the split a multiple of 32): gfx950's v_mfma_f32_16x16x32_bf16,
+// twice the BF16 rate of the 16 x 16 x 16 form, lane l holding A[l % 16][8 (l / 16) .. +7] and
+// B[8 (l / 16) .. +7][l % 16] (the same K order for both operands, so the product is the
+// GEMM's whatever order the hardware sums in).  The forward pass and the fused inference use it.
+template <bool TA, bool TB, int KS = 16>
 __global__ __launch_bounds__(256) void k_gemm_bf16(const uint16_t *__restrict__ A,
                                                    const uint16_t *__restrict__ B, float *C,
                                                    int M, int N, int K, int k_per_split,
                                                    int lda, int ldb, int epi,
                                                    const float *__restrict__ bias,
                                                    uint16_t *__restrict__ Hout) {
-  __shared__ uint16_t As[64][16 + 4];   // [m][k]
-  __shared__ uint16_t Bs[64][16 + 4];   // [n][k] (B tile stored transposed)
+  static_assert(KS == 16 || (KS == 32 && !TA && !TB), "the 32-wide step has no transposes");
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][KS + (KS == 32 ? 8 : 4)];  // [m][k]
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][KS + (KS == 32 ? 8 : 4)];  // [n][k]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
@@ -76,6 +84,32 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const uint16_t *__restrict__ 
   f32x4 acc[2][2];
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if constexpr (KS == 32) {
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+      {  // A tile [m0 .. +63][k0 .. +31], 8 per thread
+        const int r = tid >> 2, kk = (tid & 3) * 8;
+        const uint16_t *p = A + (size_t)(m0 + r) * lda + k0 + kk;
+        for (int e = 0; e < 8; e++) As[r][kk + e] = p[e];
+      }
+      {  // B tile [k0 .. +31][n0 .. +63] -> Bs[n][k], 8 per thread
+        const int kk = tid >> 3, nn = (tid & 7) * 8;
+        const uint16_t *p = B + (size_t)(k0 + kk) * ldb + n0 + nn;
+        for (int e = 0; e < 8; e++) Bs[nn + e][kk] = p[e];
+      }
+      __syncthreads();
+      const int kq = 8 * (lane >> 4);
+      for (int a = 0; a < 2; a++) {
+        const int row = wm * 32 + a * 16 + (lane & 15);
+        const bf16x8 fa = *(const bf16x8 *)&As[row][kq];
+        for (int b = 0; b < 2; b++) {
+          const int col = wn * 32 + b * 16 + (lane & 15);
+          const bf16x8 fb = *(const bf16x8 *)&Bs[col][kq];
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc[a][b], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+  } else
   for (int k0 = kb; k0 < ke; k0 += 16) {
     // A tile: op(A)[m0 .. m0+63][k0 .. k0+15]
     if (!TA) {
@@ -226,9 +260,9 @@ __global__ void k_transpose_bf16(const uint16_t *src, int K, int N, uint16_t *ds
 
 // Fused inference: every layer of the MLP for 64 rows per workgroup, activations kept in LDS as
 // BF16 (exactly what the layer-by-layer path stores between its GEMMs), weights read from the
-// transposed BF16 copies.  The MFMA sequence per output element (K in steps of 16 from 0) and
-// the epilogue (bias, GELU, BF16 rounding) are those of k_gemm_bf16, so the outputs equal the
-// layer-by-layer forward bit for bit.  Limits: padded input width and hidden width <= 128,
+// transposed BF16 copies.  The MFMA sequence per output element (gfx950's 16 x 16 x 32 form, K in
+// steps of 32 from 0) and the epilogue (bias, GELU, BF16 rounding) are those of the forward
+// k_gemm_bf16, so the outputs equal the layer-by-layer forward bit for bit.  Limits: padded input width and hidden width <= 128,
 // hidden width a multiple of 64, d_out <= 16.
 constexpr int kInferW = 128;
 constexpr int kInferMaxLayers = 6;
@@ -245,10 +279,10 @@ struct InferArgs {
 template <int K, int HID>
 __device__ __forceinline__ void infer_layer_w(const uint16_t (*Hin)[kInferW + 8],
                                               uint16_t (*Hout)[kInferW + 8],
-                                              const bf16x4 (&fb)[K / 16][HID / 64],
+                                              const bf16x8 (&fb)[K / 32][HID / 64],
                                               const float (&bc)[HID / 64], int w, int lr,
                                               int lq) {
-  constexpr int NBW = HID / 64, KS = K / 16;
+  constexpr int NBW = HID / 64, KS = K / 32;
   f32x4 acc[4][NBW];
 #pragma unroll
   for (int m = 0; m < 4; m++)
@@ -256,14 +290,14 @@ __device__ __forceinline__ void infer_layer_w(const uint16_t (*Hin)[kInferW + 8]
     for (int b = 0; b < NBW; b++) acc[m][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
-    bf16x4 fa[4];
+    bf16x8 fa[4];
 #pragma unroll
-    for (int m = 0; m < 4; m++) fa[m] = *(const bf16x4 *)&Hin[m * 16 + lr][ks * 16 + lq];
+    for (int m = 0; m < 4; m++) fa[m] = *(const bf16x8 *)&Hin[m * 16 + lr][ks * 32 + 2 * lq];
 #pragma unroll
     for (int b = 0; b < NBW; b++)
 #pragma unroll
       for (int m = 0; m < 4; m++)
-        acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[m], fb[ks][b], acc[m][b], 0, 0, 0);
+        acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[m], fb[ks][b], acc[m][b], 0, 0, 0);
   }
 #pragma unroll
   for (int b = 0; b < NBW; b++) {
@@ -277,14 +311,15 @@ __device__ __forceinline__ void infer_layer_w(const uint16_t (*Hin)[kInferW + 8]
 
 // The wave's B fragments and biases of one hidden layer.
 template <int K, int HID>
-__device__ __forceinline__ void infer_frags(bf16x4 (&fb)[K / 16][HID / 64],
+__device__ __forceinline__ void infer_frags(bf16x8 (&fb)[K / 32][HID / 64],
                                             float (&bc)[HID / 64], const uint16_t *wt,
                                             const float *bias, int w, int lr, int lq) {
 #pragma unroll
-  for (int ks = 0; ks < K / 16; ks++)
+  for (int ks = 0; ks < K / 32; ks++)
 #pragma unroll
     for (int b = 0; b < HID / 64; b++)
-      fb[ks][b] = *(const bf16x4 *)(wt + (size_t)(w * (HID / 4) + b * 16 + lr) * K + ks * 16 + lq);
+      fb[ks][b] = *(const bf16x8 *)(wt + (size_t)(w * (HID / 4) + b * 16 + lr) * K + ks * 32 +
+                                    2 * lq);
 #pragma unroll
   for (int b = 0; b < HID / 64; b++) bc[b] = bias[w * (HID / 4) + b * 16 + lr];
 }
@@ -294,7 +329,7 @@ template <int K, int HID>
 __device__ __forceinline__ void infer_layer(const uint16_t (*Hin)[kInferW + 8],
                                             uint16_t (*Hout)[kInferW + 8], const uint16_t *wt,
                                             const float *bias, int w, int lr, int lq) {
-  bf16x4 fb[K / 16][HID / 64];
+  bf16x8 fb[K / 32][HID / 64];
   float bc[HID / 64];
   infer_frags<K, HID>(fb, bc, wt, bias, w, lr, lq);
   infer_layer_w<K, HID>(Hin, Hout, fb, bc, w, lr, lq);
@@ -321,16 +356,17 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a) {
     cur ^= 1;
   }
   // output layer, columns 0..15; wave w takes rows 16w .. 16w+15
-  constexpr int KS = HID / 16;
+  constexpr int KS = HID / 32;
   const uint16_t *wt = a.wt[a.layers];
-  bf16x4 fb[KS];
+  bf16x8 fb[KS];
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) fb[ks] = *(const bf16x4 *)(wt + (size_t)lr * HID + ks * 16 + lq);
+  for (int ks = 0; ks < KS; ks++)
+    fb[ks] = *(const bf16x8 *)(wt + (size_t)lr * HID + ks * 32 + 2 * lq);
   f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
-    const bf16x4 fa = *(const bf16x4 *)&Hs[cur][w * 16 + lr][ks * 16 + lq];
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb[ks], acc, 0, 0, 0);
+    const bf16x8 fa = *(const bf16x8 *)&Hs[cur][w * 16 + lr][ks * 32 + 2 * lq];
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[ks], acc, 0, 0, 0);
   }
   if (lr < a.dout) {
     const float bc = a.b[a.layers][lr];
@@ -353,14 +389,15 @@ __global__ __launch_bounds__(256) void k_mlp_infer_p(InferArgs a) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lq = 4 * (lane >> 4);
-  constexpr int KH = HID / 16, NE = DINP / 4;
-  bf16x4 f0[DINP / 16][HID / 64], fh[NH][KH][HID / 64], fo[KH];
+  constexpr int KH = HID / 32, NE = DINP / 4;
+  bf16x8 f0[DINP / 32][HID / 64], fh[NH][KH][HID / 64], fo[KH];
   float c0[HID / 64], ch[NH][HID / 64];
   infer_frags<DINP, HID>(f0, c0, a.wt[0], a.b[0], w, lr, lq);
 #pragma unroll
   for (int l = 1; l < NH; l++) infer_frags<HID, HID>(fh[l - 1], ch[l - 1], a.wt[l], a.b[l], w, lr, lq);
 #pragma unroll
-  for (int ks = 0; ks < KH; ks++) fo[ks] = *(const bf16x4 *)(a.wt[NH] + (size_t)lr * HID + ks * 16 + lq);
+  for (int ks = 0; ks < KH; ks++)
+    fo[ks] = *(const bf16x8 *)(a.wt[NH] + (size_t)lr * HID + ks * 32 + 2 * lq);
   const float co = lr < a.dout ? a.b[NH][lr] : 0.f;
   for (int i = tid; i < 64 * (kInferW + 8); i += 256) (&Hs[2][0][0])[i] = 0;
   // this thread's input slots: row tid / 4, columns tid % 4 + 4 e
@@ -399,8 +436,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer_p(InferArgs a) {
     f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < KH; ks++) {
-      const bf16x4 fa = *(const bf16x4 *)&Hs[cur][w * 16 + lr][ks * 16 + lq];
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fo[ks], acc, 0, 0, 0);
+      const bf16x8 fa = *(const bf16x8 *)&Hs[cur][w * 16 + lr][ks * 32 + 2 * lq];
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fo[ks], acc, 0, 0, 0);
     }
     if (lr < a.dout) {
 #pragma unroll
@@ -484,7 +521,10 @@ int gemm(nb_mlp *m, bool ta, bool tb, const uint16_t *A, const uint16_t *B, floa
     epi = EPI_STORE;
   }
   dim3 grid((unsigned)(N / 64), (unsigned)(M / 64), (unsigned)splits);
-  if (!ta && !tb)
+  if (!ta && !tb && K % 32 == 0 && kps % 32 == 0)  // every forward GEMM (padded K, no split)
+    hipLaunchKernelGGL((k_gemm_bf16<false, false, 32>), grid, dim3(256), 0, m->stream, A, B, out,
+                       M, N, K, kps, lda, ldb, epi, bias, H);
+  else if (!ta && !tb)
     hipLaunchKernelGGL((k_gemm_bf16<false, false>), grid, dim3(256), 0, m->stream, A, B, out, M,
                        N, K, kps, lda, ldb, epi, bias, H);
   else if (ta && !tb)
