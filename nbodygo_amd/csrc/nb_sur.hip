@@ -460,26 +460,31 @@ __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int
     for (int q = 0; q < 10; q++) c3[q] = 0.f;
 #pragma unroll
     for (int q = 0; q < 15; q++) e4[q] = 0.f;
-    const int per = ms.width / 4;
-    const int s0 = sl * ms.width + wave * per, s1 = min(s0 + per, nec);
-    for (int sb = s0; sb < s1; sb += 64) {
-      const int cnt = min(64, s1 - sb);
-      // padding records are all zero: no mass, no quadrupole, no contribution
-      for (int q = 0; q < 4; q++)
-        rl[lane][q] = lane < cnt ? src4[(size_t)(sb + lane) * 4 + q]
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
-      wave_lds_sync();
-#pragma unroll 2
-      for (int j = 0; j < cnt; j++) {
+    // The target group's rows: nonempty cells are in cell order (x fastest, then y, then z), so
+    // z runs from the first lane's to the last valid lane's, and so does y when z is constant.
+    const int tlast = min(63, nec - 1 - tg * 64);
+    auto lane_f = [](float v, int l) {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    };
+    const float tza = lane_f(iz, 0), tzb = lane_f(iz, tlast);
+    const float tya = tza == tzb ? lane_f(iy, 0) : -1e30f;
+    const float tyb = tza == tzb ? lane_f(iy, tlast) : 1e30f;
+    // One source cell: its monopole's sums and its quadrupole's field.  With TEST the cell is
+    // skipped when it lies within Chebyshev distance K of the target (the per-body shell has it);
+    // without, the caller has shown that every pair of the chunk lies beyond.
+    auto pair = [&](const int j, auto test) {
         const float4 r0 = rl[j][0], r1 = rl[j][1], r2v = rl[j][2], r3 = rl[j][3];
         // r0 = COM, M; r1 = Qxx Qyy Qzz Qxy; r2 = Qxz Qyz radius ix; r3 = iy iz n 0
-        const float cheb = fmaxf(fabsf(r2v.w - ix), fmaxf(fabsf(r3.x - iy), fabsf(r3.y - iz)));
         const float rx = X - r0.x, ry = Y - r0.y, rz = Z - r0.z;
         float rr = __builtin_fmaf(rx, rx, eps2);
         rr = __builtin_fmaf(ry, ry, rr);
         rr = __builtin_fmaf(rz, rz, rr);
         float inv = __builtin_amdgcn_rsqf(rr);
-        inv = cheb > Kf ? inv : 0.f;
+        if constexpr (decltype(test)::value) {
+          const float cheb =
+              fmaxf(fabsf(r2v.w - ix), fmaxf(fabsf(r3.x - iy), fabsf(r3.y - iz)));
+          inv = cheb > Kf ? inv : 0.f;
+        }
         const float inv2 = inv * inv, M = r0.w;
         const float m3 = M * inv2 * inv, m5 = m3 * inv2, m7 = m5 * inv2, m9 = m7 * inv2;
         // quadrupole field
@@ -546,6 +551,27 @@ __global__ __launch_bounds__(256) void k_sur_m2l(const float *crec_ne, const int
         e4[12] = __builtin_fmaf(eyy, zz, e4[12]);
         e4[13] = __builtin_fmaf(ezz, yz, e4[13]);
         e4[14] = __builtin_fmaf(ezz, zz, e4[14]);
+    };
+    const int per = ms.width / 4;
+    const int s0 = sl * ms.width + wave * per, s1 = min(s0 + per, nec);
+    for (int sb = s0; sb < s1; sb += 64) {
+      const int cnt = min(64, s1 - sb);
+      // padding records are all zero: no mass, no quadrupole, no contribution
+      for (int q = 0; q < 4; q++)
+        rl[lane][q] = lane < cnt ? src4[(size_t)(sb + lane) * 4 + q]
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      wave_lds_sync();
+      // the chunk's rows, as for the targets; a gap above K + 1/2 in z or y puts every pair
+      // beyond the shell (Chebyshev distance is at least the gap on any one axis)
+      const float4 fa = rl[0][3], fb = rl[cnt - 1][3];
+      const float sya = fa.y == fb.y ? fa.x : -1e30f, syb = fa.y == fb.y ? fb.x : 1e30f;
+      const float gap = fmaxf(fmaxf(fa.y - tzb, tza - fb.y), fmaxf(sya - tyb, tya - syb));
+      if (__builtin_amdgcn_readfirstlane(gap > Kf)) {
+#pragma unroll 2
+        for (int j = 0; j < cnt; j++) pair(j, std::false_type{});
+      } else {
+#pragma unroll 2
+        for (int j = 0; j < cnt; j++) pair(j, std::true_type{});
       }
       wave_lds_sync();
     }
