@@ -759,17 +759,17 @@ __device__ void exact_chunk_pairs(const NearArgs &a, uint32_t i, int s0, int cnt
 }
 
 // One workgroup (4 waves) per work item = up to 64 bodies of one cell, one lane each.  The waves
-// split the cell's neighbourhood: 64-source chunks of the nine contiguous rows of adjacent cells
-// (round robin over the chunks) and the shell cells (round robin), streaming both through the
-// scalar cache; partial sums meet in LDS in wave order, and wave 0 adds the Taylor step of the
-// far expansion and writes the base acceleration and the MLP features.
+// split the cell's neighbourhood: four equal contiguous shares of the nine rows of adjacent cells,
+// walked in chunks of 64 sources (128 with collisions) staged in the wave's LDS tile, and the
+// shell cells (round robin, staged by ballot); partial sums meet in LDS in wave order, and wave 0
+// adds the Taylor step of the far expansion and writes the base acceleration and the MLP features.
 // COLL: each lane also keeps the minimum r^2 of every chunk and compares it against a bound that
 // no overlapping pair can exceed in FP32 (the chunk's largest radius and coordinate, rounded up,
 // plus the FP32 rounding of positions); a flagged (body, chunk) is re-tested exactly in FP64.
-// LDS: one 3 KB region per wave, used in turn for the wave's source tile (64 float4), its staged
-// shell records (64 x 3 float4) and, in waves 1-3, its partial sums for wave 0 (3 x 64 doubles):
-// 12 KB per workgroup.  Without collisions the kernel is held to 64 VGPRs; with both, 8 waves
-// fit per SIMD.
+// LDS: one 3 KB region per wave, used in turn for the wave's source tile (64 float4, 128 with
+// collisions), its staged shell records (64 x 3 float4) and, in waves 1-3, its partial sums for
+// wave 0 (3 x 64 doubles): 12 KB per workgroup.  Without collisions the kernel is held to 64
+// VGPRs, so 8 waves fit per SIMD; with collisions it takes 94 VGPRs, 5 waves per SIMD.
 template <bool COLL>
 __device__ __forceinline__ void near_item(const NearArgs &a, const int item,
                                           float4 (*wlds)[64 * 3]) {
@@ -842,6 +842,9 @@ __device__ __forceinline__ void near_item(const NearArgs &a, const int item,
   int cell0 = 0;
   const bool in0 = shell_pos(4 * lane + wave, cell0);
   const int cnt0 = in0 ? a.ccount[cell0] : 0;
+  // sources per chunk: a chunk's fixed costs (staging, the collision bound's reductions, the
+  // FP64 accumulation) are spread over twice the pairs with collisions on
+  constexpr int CH = COLL ? 128 : 64;
   const int share = (nsrc + 3) / 4;
   const int w0 = wave * share, w1 = min(nsrc, w0 + share);
   int base = 0;
@@ -849,25 +852,29 @@ __device__ __forceinline__ void near_item(const NearArgs &a, const int item,
     int rb, rlen;
     row_range(q, rb, rlen);
     const int lo = max(w0, base), hi = min(w1, base + rlen);
-    for (int s = lo; s < hi; s += 64) {
+    for (int s = lo; s < hi; s += CH) {
       {
         const int s0 = rb + (s - base);
-        const int cnt = min(64, hi - s);
-        // the wave's private tile: one source per lane, zero-mass padding to a multiple of 4
-        const float4 src = lane < cnt ? a.sp[s0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-        tl[lane] = src;
+        const int cnt = min(CH, hi - s);
+        // the wave's private tile: CH / 64 sources per lane, zero-mass padding to a multiple of 4
         float rmax_c = 0.f, xmax_c = 0.f;
-        if (COLL) {
-          if (lane < cnt) {
-            rmax_c = round_up_f(a.r[a.perm[s0 + lane]]);
-            xmax_c = fmaxf(fabsf(src.x), fmaxf(fabsf(src.y), fabsf(src.z)));
+#pragma unroll
+        for (int h = 0; h < CH; h += 64) {
+          const int e = h + lane;
+          const float4 src = e < cnt ? a.sp[s0 + e] : make_float4(0.f, 0.f, 0.f, 0.f);
+          tl[e] = src;
+          if (COLL && e < cnt) {
+            rmax_c = fmaxf(rmax_c, round_up_f(a.r[a.perm[s0 + e]]));
+            xmax_c = fmaxf(xmax_c, fmaxf(fabsf(src.x), fmaxf(fabsf(src.y), fabsf(src.z))));
           }
+        }
+        if (COLL) {
           rmax_c = wave_maxf(rmax_c);
           xmax_c = wave_maxf(xmax_c);
         }
         wave_lds_sync();
         float px = 0, py = 0, pz = 0, r2min = FLT_MAX;
-        // the tile padded with zero-mass entries to a multiple of 4 copies (at most 64, the
+        // the tile padded with zero-mass entries to a multiple of 4 copies (at most CH, the
         // tile's size); copy cp takes the contiguous share [cp len, (cp + 1) len), four sources
         // per step, with the same trip count on every lane
         const int q4 = 4 * copies;
