@@ -769,7 +769,8 @@ __device__ void exact_chunk_pairs(const NearArgs &a, uint32_t i, int s0, int cnt
 // LDS: one 3 KB region per wave, used in turn for the wave's source tile (64 float4, 128 with
 // collisions), its staged shell records (64 x 3 float4) and, in waves 1-3, its partial sums for
 // wave 0 (3 x 64 doubles): 12 KB per workgroup.  Without collisions the kernel is held to 64
-// VGPRs, so 8 waves fit per SIMD; with collisions it takes 94 VGPRs, 5 waves per SIMD.
+// VGPRs, so 8 waves fit per SIMD; with collisions it is held to 80 VGPRs (6 waves per SIMD, no
+// spills; 94 VGPRs and 5 waves unconstrained).
 template <bool COLL>
 __device__ __forceinline__ void near_item(const NearArgs &a, const int item,
                                           float4 (*wlds)[64 * 3]) {
@@ -1064,7 +1065,7 @@ __device__ __forceinline__ void near_item(const NearArgs &a, const int item,
 // further items; the barrier between two items keeps wave 0's reads of the partial sums ahead of
 // the other waves' next writes).
 template <bool COLL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COLL ? 1 : 8)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COLL ? 6 : 8)))
 void k_sur_near(NearArgs a) {
   __shared__ float4 wlds[4][64 * 3];
   const int nitems = *a.nitems;
